@@ -1,0 +1,196 @@
+/*
+ * whir_amd.h -- C ABI of libwhir_amd, the MI355X (gfx950) WHIR prover hot path.
+ *
+ * Every entry point returns 0 on success and a nonzero status on failure; the
+ * message is then available from whir_last_error() (per calling thread).  No
+ * C++ exception crosses this boundary.  A binding turns a nonzero status into
+ * the reference's panic (the reference asserts on contract violations, e.g.
+ * src/algebra/ntt/cooley_tukey.rs:412-423).
+ *
+ * Field elements are canonical (non-Montgomery) little-endian uint64 words,
+ * `degree` words per element, lowest coefficient first: degree 1 = Field64,
+ * 2 = Field64_2 (u^2 = 7), 3 = Field64_3 (u^3 = 2) (src/algebra/fields.rs:85-143).
+ * A binding over ark-ff converts with into_bigint/from_bigint at the boundary.
+ *
+ * Host-pointer entry points copy in, run on the GPU and copy out before they
+ * return; nothing retains a host pointer.  `_dev` entry points take device
+ * pointers (from whir_dev_alloc or any HIP allocation), run on the calling
+ * thread's stream and return once the work is enqueued unless they return a
+ * host value; whir_synchronize() waits for the calling thread's stream.
+ * Every entry point is thread-safe; each host thread has its own stream.
+ *
+ * Hash engines (src/hash/mod.rs:28-36): 0 Copy, 1 Sha2 (SHA-256), 2 Keccak
+ * (Keccak-256), 3 Sha3 (SHA3-256), 4 Blake3.
+ */
+#ifndef WHIR_AMD_H
+#define WHIR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WHIR_HASH_COPY = 0,
+  WHIR_HASH_SHA2 = 1,
+  WHIR_HASH_KECCAK = 2,
+  WHIR_HASH_SHA3 = 3,
+  WHIR_HASH_BLAKE3 = 4
+};
+
+/* ----------------------------------------------------------------- runtime */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *whir_last_error(void);
+/* ABI version of this library. */
+int whir_abi_version(void);
+/* Number of visible GPUs (0 when no HIP device is present). */
+int whir_device_count(int *count);
+/* Select the device for the calling thread. */
+int whir_set_device(int device);
+
+int whir_dev_alloc(size_t bytes, void **ptr);
+int whir_dev_free(void *ptr);
+int whir_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int whir_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int whir_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* Wait for all work enqueued by the calling thread. */
+int whir_synchronize(void);
+
+/* -------------------------------------------- ReedSolomon<F> (interleaved RS)
+ * Replaces the trait ReedSolomon<F> (src/algebra/ntt/mod.rs:65-100) and the free
+ * functions next_order / generator / evaluation_points / interleaved_rs_encode
+ * (src/algebra/ntt/mod.rs:104-134), implemented by NttEngine
+ * (src/algebra/ntt/cooley_tukey.rs:363-477). */
+
+/* next_order (cooley_tukey.rs:364-368): smallest divisor of 3*2^32 >= size.
+ * Returns a nonzero status when size exceeds the largest supported order. */
+int whir_rs_next_order(uint64_t size, uint64_t *out);
+
+/* generator (cooley_tukey.rs:370-373): 7^((p-1)/codeword_length) (base field). */
+int whir_rs_generator(uint64_t codeword_length, uint64_t *out);
+
+/* evaluation_points (cooley_tukey.rs:375-402), default (permuted) order:
+ * point of index r = c*C + t is generator^(c + t*Q). */
+int whir_rs_evaluation_points(uint64_t masked_message_length, uint64_t codeword_length,
+                              const uint64_t *indices, size_t count, uint64_t *out);
+
+/* interleaved_encode (cooley_tukey.rs:411-477): messages[j] points to
+ * message_length elements; masks is num_messages x mask_length elements (may be
+ * NULL when mask_length == 0); out receives codeword_length x num_messages
+ * elements, row-major (row r holds the r-th evaluation of every message).
+ * The GPU path needs power-of-two codeword_length and coset size. */
+int whir_rs_interleaved_encode(int degree, const uint64_t *const *messages, size_t num_messages,
+                               size_t message_length, const uint64_t *masks, size_t mask_length,
+                               size_t codeword_length, uint64_t *out);
+
+/* Device variant.  messages: num_messages contiguous messages of message_length
+ * elements; out: codeword_length x num_messages elements, column-major
+ * (out[j*codeword_length + r]) when row_major == 0, row-major otherwise. */
+int whir_rs_interleaved_encode_dev(int degree, const uint64_t *messages, size_t num_messages,
+                                   size_t message_length, const uint64_t *masks, size_t mask_length,
+                                   size_t codeword_length, uint64_t *out, int row_major);
+
+/* ------------------------------------------------------------- HashEngine
+ * Replaces HashEngine (src/hash/mod.rs:58-86) for the engines of the registry
+ * (src/hash/mod.rs:28-36): Blake3 (src/hash/blake3_engine.rs:83-160), Sha2 / Sha3 /
+ * Keccak (src/hash/digest_engine.rs:102-142), Copy (src/hash/copy_engine.rs). */
+
+/* engine_id (src/hash/mod.rs:88-103): SHA3-256("whir::hash" || oid or name). */
+int whir_hash_engine_id(int engine, uint8_t out[32]);
+/* supports_size (src/hash/mod.rs:65-66).  *out = 1 if supported. */
+int whir_hash_supports_size(int engine, size_t size, int *out);
+/* hash_many (src/hash/mod.rs:79-85): `count` messages of `size` bytes, concatenated. */
+int whir_hash_many(int engine, size_t size, const uint8_t *input, size_t count, uint8_t *output);
+int whir_hash_many_dev(int engine, size_t size, const uint8_t *input, size_t count, uint8_t *output);
+
+/* --------------------------------------------------- matrix / Merkle commit
+ * Replace the bodies of matrix_commit::Config::commit (src/protocols/matrix_commit.rs:211-232,
+ * leaf hashing hash_rows 301-368) and merkle_tree::Config::commit / open
+ * (src/protocols/merkle_tree.rs:82-186). */
+
+/* Number of hashes in the witness of a tree over num_leaves leaves:
+ * 2^(layers_for_size(num_leaves)+1) - 1 (merkle_tree.rs:77-79, 281-283). */
+int whir_merkle_num_nodes(size_t num_leaves, size_t *out);
+
+/* merkle_tree::Config::commit: leaves = num_leaves x 32 bytes; nodes receives
+ * whir_merkle_num_nodes hashes laid out [padded leaves][layer]...[root]. */
+int whir_merkle_commit(int engine, const uint8_t *leaves, size_t num_leaves, uint8_t *nodes);
+int whir_merkle_commit_dev(int engine, uint8_t *nodes, size_t num_leaves);
+
+/* matrix_commit::Config::commit: hashes each row of a row-major rows x cols
+ * matrix of degree-`degree` elements (ArkFieldEncoder bytes) and builds the tree. */
+int whir_matrix_commit(int engine, int degree, const uint64_t *matrix, size_t rows, size_t cols,
+                       uint8_t *nodes);
+
+/* Leaf hashes of a device codeword in column-major layout (cw[j*rows + r]). */
+int whir_hash_rows_cols_dev(int engine, int degree, const uint64_t *cw, size_t rows, size_t cols,
+                            uint8_t *leaves);
+
+/* merkle_tree::Config::open (merkle_tree.rs:145-186): writes the sibling hints
+ * (32 bytes each) in transcript order; *num_hints receives their count.  hints
+ * must hold count * layers_for_size(num_leaves) hashes. */
+int whir_merkle_open(const uint8_t *nodes, size_t num_leaves, const size_t *indices, size_t count,
+                     uint8_t *hints, size_t *num_hints);
+
+/* --------------------------------------------------- sumcheck and covectors
+ * Replace src/algebra/sumcheck.rs:12-111, src/algebra/multilinear.rs:170-190 and
+ * src/algebra/mod.rs:36-177.  Vectors hold n elements of `degree` words. */
+
+/* compute_sumcheck_polynomial (sumcheck.rs:12-63): out = (c0, c2), 2*degree words. */
+int whir_sumcheck_polynomial(int degree, const uint64_t *a, const uint64_t *b, size_t n, uint64_t *out);
+int whir_sumcheck_polynomial_dev(int degree, const uint64_t *a, const uint64_t *b, size_t n, uint64_t *out);
+/* fold (sumcheck.rs:69-104), in place; *new_len = next_pow2(n)/2 (n if n <= 1). */
+int whir_fold(int degree, uint64_t *values, size_t n, const uint64_t *weight, size_t *new_len);
+int whir_fold_dev(int degree, uint64_t *values, size_t n, const uint64_t *weight, size_t *new_len);
+/* fold_and_compute_polynomial (sumcheck.rs:106-111), one fused pass. */
+int whir_fold_and_compute_polynomial(int degree, uint64_t *a, uint64_t *b, size_t n, const uint64_t *weight,
+                                     uint64_t *out, size_t *new_len);
+int whir_fold_and_compute_polynomial_dev(int degree, uint64_t *a, uint64_t *b, size_t n,
+                                         const uint64_t *weight, uint64_t *out, size_t *new_len);
+/* eval_eq (multilinear.rs:170-190): acc has 2^k elements, point k elements. */
+int whir_eval_eq(int degree, uint64_t *acc, const uint64_t *point, size_t k, const uint64_t *scalar);
+int whir_eval_eq_dev(int degree, uint64_t *acc, const uint64_t *point, size_t k, const uint64_t *scalar);
+/* geometric_accumulate (algebra/mod.rs:154-177): acc[i] += sum_j scalars[j] points[j]^i. */
+int whir_geometric_accumulate(int degree, uint64_t *acc, size_t n, const uint64_t *scalars,
+                              const uint64_t *points, size_t m);
+int whir_geometric_accumulate_dev(int degree, uint64_t *acc, size_t n, const uint64_t *scalars,
+                                  const uint64_t *points, size_t m);
+/* mixed_univariate_evaluate (algebra/mod.rs:105-129): coefficients of degree
+ * src_degree (1 = base field, or = degree), point of `degree` words. */
+int whir_mixed_univariate_evaluate(int src_degree, int degree, const uint64_t *coeffs, size_t n,
+                                   const uint64_t *point, uint64_t *out);
+int whir_mixed_univariate_evaluate_dev(int src_degree, int degree, const uint64_t *coeffs, size_t n,
+                                       const uint64_t *point, uint64_t *out);
+/* lift (algebra/mod.rs:51-59): base-field src -> degree-`degree` dst. */
+int whir_lift(int degree, const uint64_t *src, size_t n, uint64_t *dst);
+int whir_lift_dev(int degree, const uint64_t *src, size_t n, uint64_t *dst);
+/* mixed_scalar_mul_add (algebra/mod.rs:88-98): acc[i] += weight * v[i], v base field. */
+int whir_mixed_scalar_mul_add(int degree, uint64_t *acc, const uint64_t *weight, const uint64_t *v, size_t n);
+int whir_mixed_scalar_mul_add_dev(int degree, uint64_t *acc, const uint64_t *weight, const uint64_t *v,
+                                  size_t n);
+/* dot (algebra/mod.rs:36). */
+int whir_dot(int degree, const uint64_t *a, const uint64_t *b, size_t n, uint64_t *out);
+
+/* ------------------------------------------------------------ proof of work
+ * Replaces the grinding loop of proof_of_work::Config::prove
+ * (src/protocols/proof_of_work.rs:86-163). */
+
+/* threshold (proof_of_work.rs:29-39). */
+int whir_pow_threshold(double difficulty_bits, uint64_t *out);
+/* Minimal nonce with u64_le(H(challenge || nonce_le || 0^24)[0..8]) <= threshold.
+ * threshold == UINT64_MAX is the reference's no-op: *nonce = 0, *performed = 0. */
+int whir_pow_grind(int engine, const uint8_t challenge[32], uint64_t threshold, uint64_t *nonce,
+                   int *performed);
+/* Same search restricted to nonces below max_nonces (one bounded launch);
+ * *found = 0 when none of them meets the threshold. */
+int whir_pow_grind_bounded(int engine, const uint8_t challenge[32], uint64_t threshold, uint64_t max_nonces,
+                           uint64_t *nonce, int *found);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WHIR_AMD_H */

@@ -1,0 +1,49 @@
+// Device sumcheck and covector kernels (algebra.hip).  Vectors are arrays of
+// degree-D extension elements (D u64 words each, canonical, lowest coefficient
+// first).  Scalars passed by value are host arrays of D words.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace whir {
+
+// compute_sumcheck_polynomial (src/algebra/sumcheck.rs:12-63) over a[0..n), b[0..n);
+// result (c0, c2) written to dev_out[0..2D) on the device.
+void sumcheck_poly_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* dev_out, hipStream_t st);
+
+// fold (src/algebra/sumcheck.rs:69-104), in place; returns the new length.
+size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t st);
+
+// fold a and b by w, then compute_sumcheck_polynomial of the folded vectors, in one
+// pass (src/algebra/sumcheck.rs:106-111).  Returns the new length.
+size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* dev_out,
+                         hipStream_t st);
+
+// eval_eq (src/algebra/multilinear.rs:170-190): acc[x] += s * prod_i eq(z_i, x_i), x_0 = MSB.
+void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t st);
+
+// geometric_accumulate (src/algebra/mod.rs:154-177): acc[i] += sum_j s_j x_j^i.
+void geometric_accumulate_dev(int D, uint64_t* acc, size_t n, const uint64_t* scalars, const uint64_t* points,
+                              size_t m, hipStream_t st);
+
+// mixed_univariate_evaluate (src/algebra/mod.rs:105-129): sum_i c_i x^i with c in the
+// base field (Ds = 1) or in the extension (Ds = D); result to dev_out[0..D).
+void univariate_eval_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, uint64_t* dev_out,
+                         hipStream_t st);
+
+// lift (src/algebra/mod.rs:51-59): dst[i] = embed(src[i]).
+void lift_dev(int D, const uint64_t* src, size_t n, uint64_t* dst, hipStream_t st);
+
+// mixed_scalar_mul_add (src/algebra/mod.rs:88-98): acc[i] += w * v[i], v in the base field.
+void mixed_scalar_mul_add_dev(int D, uint64_t* acc, const uint64_t* w, const uint64_t* v, size_t n, hipStream_t st);
+
+// dot (src/algebra/mod.rs:36) to dev_out[0..D).
+void dot_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* dev_out, hipStream_t st);
+
+// out[k] = v[idx[k]] rows of a column-major matrix (gather for openings):
+// out[k*M + j] = cw[j*N + idx[k]] (degree-D elements).
+void gather_rows_dev(int D, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx, size_t k, uint64_t* out,
+                     hipStream_t st);
+
+}  // namespace whir

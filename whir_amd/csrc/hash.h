@@ -1,0 +1,39 @@
+// Device hash engines, leaf hashing and Merkle tree construction (hash.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace whir {
+
+// Engine numbering of the C ABI; the reference's registry is src/hash/mod.rs:28-36.
+enum Engine : int { ENG_COPY = 0, ENG_SHA2 = 1, ENG_KECCAK = 2, ENG_SHA3 = 3, ENG_BLAKE3 = 4 };
+
+bool engine_supports_size(int engine, size_t size);
+
+// count messages of `size` bytes each (contiguous) -> count 32-byte digests.
+void hash_many_dev(int engine, size_t size, const uint8_t* in, size_t count, uint8_t* out, hipStream_t st);
+
+// Leaf hashes of a column-major codeword cw[j*N + r] (M columns of degree-D
+// elements): leaf r = H(canonical LE bytes of row r).
+void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves,
+                        hipStream_t st);
+
+// Leaf hashes of a row-major matrix m[r*M + j].
+void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, uint8_t* leaves,
+                   hipStream_t st);
+
+// Merkle nodes over the first num_leaves leaf hashes already stored in
+// nodes[0 .. num_leaves); nodes holds (2^(layers+1) - 1) hashes.
+size_t merkle_layers(size_t num_leaves);
+size_t merkle_num_nodes(size_t num_leaves);
+void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st);
+
+// Proof of work: the minimal nonce in [first, first + count) with
+// u64_le(H(chal || nonce_le || 0^24)[0..8]) <= threshold; false if there is none.
+// first and count are multiples of pow_chunk().  One bounded launch.
+bool pow_grind_dev(int engine, const uint8_t challenge[32], uint64_t threshold, uint64_t first, uint64_t count,
+                   uint64_t* nonce, hipStream_t st);
+uint64_t pow_chunk();
+
+}  // namespace whir

@@ -1,0 +1,303 @@
+// Leaf hashing, Merkle layers and proof-of-work grinding on CDNA4.
+//
+// hash_many        HashEngine::hash_many (src/hash/mod.rs:79-85), one message per lane.
+// leaves (cols)    matrix_commit::hash_rows (src/protocols/matrix_commit.rs:301-368) over the
+//                  column-major codeword: a block stages RB whole rows in LDS with one
+//                  coalesced read per column, then each lane hashes one row.  Field
+//                  elements are canonical, so the ArkFieldEncoder bytes (canonical
+//                  little-endian u64 per base coefficient, matrix_commit.rs:130-156) are
+//                  the words as stored.
+// merkle           merkle_tree::Config::commit (src/protocols/merkle_tree.rs:82-127): each
+//                  block reduces a 2^k-node slice of one level through k levels in LDS,
+//                  writing every node it makes to its place in the nodes array.
+// pow              proof_of_work::Config::prove (src/protocols/proof_of_work.rs:68-166):
+//                  persistent search over nonce chunks handed out in increasing order; the
+//                  result is the globally minimal nonce.
+#include "common.h"
+#include "hash.h"
+#include "hash_dev.h"
+
+#include <algorithm>
+
+namespace whir {
+
+bool engine_supports_size(int engine, size_t size) {
+  switch (engine) {
+    case ENG_COPY: return size <= 32;
+    case ENG_SHA2:
+    case ENG_KECCAK:
+    case ENG_SHA3:
+    case ENG_BLAKE3: return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void store_digest(uint8_t* out, const uint32_t h[8]) {
+  uint4* o = (uint4*)out;
+  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+__global__ void k_hash_many(int engine, size_t size, const uint8_t* __restrict__ in, size_t count,
+                            uint8_t* __restrict__ out) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  hd::MemReader rd{in + i * size, size};
+  uint32_t h[8];
+  hd::hash_msg(engine, rd, size, h);
+  store_digest(out + 32 * i, h);
+}
+
+void hash_many_dev(int engine, size_t size, const uint8_t* in, size_t count, uint8_t* out, hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, size), "hash_many: engine does not support this message size");
+  if (!count) return;
+  k_hash_many<<<blocks_for(count, 128), 128, 0, st>>>(engine, size, in, count, out);
+  WHIR_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ leaves
+static constexpr int LEAF_THREADS = 128;
+
+template <int D>
+__global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(int engine, const uint64_t* __restrict__ cw, size_t N,
+                                                              size_t M, uint8_t* __restrict__ leaves, int RB) {
+  extern __shared__ uint64_t srow[];
+  const size_t W = M * D;          // words per row
+  const size_t stride = W + 1;     // padded LDS row stride (odd: spreads banks)
+  const size_t r0 = (size_t)blockIdx.x * RB;
+  const size_t nrows = (N - r0) < (size_t)RB ? (N - r0) : (size_t)RB;
+  for (size_t e = threadIdx.x; e < nrows * M; e += blockDim.x) {
+    size_t r = e % nrows, j = e / nrows;
+    const uint64_t* src = cw + (j * N + r0 + r) * D;
+#pragma unroll
+    for (int d = 0; d < D; d++) srow[r * stride + j * D + d] = src[d];
+  }
+  __syncthreads();
+  for (size_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+    hd::WordReader rd{srow + r * stride, 8 * W};
+    uint32_t h[8];
+    hd::hash_msg(engine, rd, 8 * W, h);
+    store_digest(leaves + 32 * (r0 + r), h);
+  }
+}
+
+// Rows wider than the LDS tile: one lane per row reading straight from HBM.
+template <int D>
+__global__ void k_leaves_cols_wide(int engine, const uint64_t* __restrict__ cw, size_t N, size_t M,
+                                   uint8_t* __restrict__ leaves) {
+  struct ColReader {
+    const uint64_t* cw;
+    size_t N, r, size;
+    __device__ uint64_t w64(size_t off) const {
+      if (off >= size) return 0;
+      size_t word = off >> 3, j = word / D, d = word % D;
+      return cw[(j * N + r) * D + d];
+    }
+    __device__ uint32_t w32(size_t off) const { return (uint32_t)(w64(off & ~(size_t)7) >> (8 * (off & 7))); }
+  };
+  size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  ColReader rd{cw, N, r, 8 * M * D};
+  uint32_t h[8];
+  hd::hash_msg(engine, rd, 8 * M * D, h);
+  store_digest(leaves + 32 * r, h);
+}
+
+template <int D>
+static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st) {
+  size_t W = M * D;
+  int RB = (int)std::min<size_t>(LEAF_THREADS, 8192 / (W + 1));
+  if (RB >= 16) {
+    size_t smem = (size_t)RB * (W + 1) * 8;
+    k_leaves_cols<D><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(engine, cw, N, M, leaves, RB);
+  } else {
+    k_leaves_cols_wide<D><<<blocks_for(N, 128), 128, 0, st>>>(engine, cw, N, M, leaves);
+  }
+  WHIR_HIP(hipGetLastError());
+}
+
+void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves,
+                        hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D), "leaf hash: engine does not support the row size");
+  if (!N) return;
+  switch (D) {
+    case 1: leaves_cols<1>(engine, cw, N, M, leaves, st); break;
+    case 2: leaves_cols<2>(engine, cw, N, M, leaves, st); break;
+    case 3: leaves_cols<3>(engine, cw, N, M, leaves, st); break;
+    default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
+  }
+}
+
+void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, uint8_t* leaves, hipStream_t st) {
+  hash_many_dev(engine, 8 * M * D, (const uint8_t*)m, rows, leaves, st);
+}
+
+// ------------------------------------------------------------------ Merkle
+size_t merkle_layers(size_t n) {
+  size_t l = 0;
+  while (((size_t)1 << l) < n) l++;
+  return l;
+}
+size_t merkle_num_nodes(size_t n) { return ((size_t)2 << merkle_layers(n)) - 1; }
+
+static constexpr int MERKLE_THREADS = 256;
+
+// Input level of width `w` at nodes + in_off*32; reduces `levels` levels.  Block b
+// takes input nodes [b*2^levels, (b+1)*2^levels).  Level of width x starts at 2W - 2x.
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_levels(int engine, uint8_t* __restrict__ nodes, size_t W,
+                                                                  size_t w, int levels) {
+  __shared__ uint32_t cur[MERKLE_THREADS * 2][8];
+  const size_t span = (size_t)1 << levels;
+  const size_t base = (size_t)blockIdx.x * span;  // first input node of this block (within level)
+  const uint8_t* in = nodes + 32 * (2 * W - 2 * w);
+  // Level 1: pairs straight from global memory.
+  size_t width = w;
+  size_t npairs = span >> 1;
+  for (size_t k = threadIdx.x; k < npairs; k += blockDim.x) {
+    hd::MemReader rd{in + 32 * (base + 2 * k), 64};
+    uint32_t h[8];
+    hd::hash_msg(engine, rd, 64, h);
+    size_t out_idx = (2 * W - 2 * (width >> 1)) + (base >> 1) + k;
+    store_digest(nodes + 32 * out_idx, h);
+#pragma unroll
+    for (int i = 0; i < 8; i++) cur[k][i] = h[i];
+  }
+  width >>= 1;
+  size_t lbase = base >> 1;
+  for (int l = 1; l < levels; l++) {
+    __syncthreads();
+    npairs >>= 1;
+    uint32_t h[8];
+    bool act = threadIdx.x < npairs;
+    if (act) {
+      uint64_t pair[8];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        pair[i] = (uint64_t)cur[2 * threadIdx.x][2 * i] | ((uint64_t)cur[2 * threadIdx.x][2 * i + 1] << 32);
+        pair[4 + i] = (uint64_t)cur[2 * threadIdx.x + 1][2 * i] | ((uint64_t)cur[2 * threadIdx.x + 1][2 * i + 1] << 32);
+      }
+      hd::WordReader rd{pair, 64};
+      hd::hash_msg(engine, rd, 64, h);
+      size_t out_idx = (2 * W - 2 * (width >> 1)) + (lbase >> 1) + threadIdx.x;
+      store_digest(nodes + 32 * out_idx, h);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) cur[threadIdx.x][i] = h[i];
+    }
+    width >>= 1;
+    lbase >>= 1;
+  }
+}
+
+void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, 64), "merkle: engine does not hash 64-byte nodes");
+  size_t layers = merkle_layers(num_leaves);
+  size_t W = (size_t)1 << layers;
+  if (W > num_leaves) fill_zero(nodes + 32 * num_leaves, 32 * (W - num_leaves), st);
+  size_t w = W;
+  const int maxlev = 9;  // 512 input nodes per block -> 256 lanes on the first level
+  while (w > 1) {
+    int lv = 0;
+    while (lv < maxlev && ((size_t)1 << (lv + 1)) <= w) lv++;
+    size_t blocks = w >> lv;
+    int threads = std::max(64, std::min(MERKLE_THREADS, 1 << (lv - 1)));
+    k_merkle_levels<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, lv);
+    WHIR_HIP(hipGetLastError());
+    w >>= lv;
+  }
+}
+
+// ------------------------------------------------------------------ PoW
+static constexpr int POW_THREADS = 256;
+static constexpr int POW_PER_LANE = 16;
+static constexpr uint64_t POW_CHUNK = (uint64_t)POW_THREADS * POW_PER_LANE;
+
+// state[0] = minimal nonce found (init UINT64_MAX), state[1] = next chunk index.
+__global__ __launch_bounds__(POW_THREADS) void k_pow(int engine, uint4 ch_lo, uint4 ch_hi, uint64_t threshold,
+                                                     uint64_t first, uint64_t num_chunks, unsigned long long* state) {
+  __shared__ unsigned long long s_chunk;
+  __shared__ int s_stop;
+  const uint32_t chw[8] = {ch_lo.x, ch_lo.y, ch_lo.z, ch_lo.w, ch_hi.x, ch_hi.y, ch_hi.z, ch_hi.w};
+  for (;;) {
+    if (threadIdx.x == 0) {
+      unsigned long long c = atomicAdd(&state[1], 1ull);
+      unsigned long long best = __hip_atomic_load(&state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // Stop when the chunks run out or every remaining chunk lies above the current minimum.
+      s_stop = (c >= num_chunks) || (first + c * POW_CHUNK > best);
+      s_chunk = c;
+    }
+    __syncthreads();
+    const unsigned long long chunk = s_chunk;
+    const int stop = s_stop;
+    __syncthreads();
+    if (stop) break;
+    uint64_t start = first + chunk * POW_CHUNK;
+    for (int k = 0; k < POW_PER_LANE; k++) {
+      uint64_t nonce = start + (uint64_t)k * POW_THREADS + threadIdx.x;
+      uint32_t h[8];
+      if (engine == ENG_BLAKE3) {
+        uint32_t m[16];
+#pragma unroll
+        for (int i = 0; i < 8; i++) m[i] = chw[i];
+        m[8] = (uint32_t)nonce;
+        m[9] = (uint32_t)(nonce >> 32);
+#pragma unroll
+        for (int i = 10; i < 16; i++) m[i] = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[i] = hd::B3_IV[i];
+        hd::b3_compress(h, m, 64, 0, hd::B3_CHUNK_START | hd::B3_CHUNK_END | hd::B3_ROOT);
+      } else {
+        uint64_t words[8];
+#pragma unroll
+        for (int i = 0; i < 4; i++) words[i] = (uint64_t)chw[2 * i] | ((uint64_t)chw[2 * i + 1] << 32);
+        words[4] = nonce;
+        words[5] = words[6] = words[7] = 0;
+        hd::WordReader rd{words, 64};
+        hd::hash_msg(engine, rd, 64, h);
+      }
+      uint64_t v = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
+      if (v <= threshold) atomicMin(&state[0], (unsigned long long)nonce);
+    }
+  }
+}
+
+__global__ void k_pow_init(unsigned long long* state) {
+  if (threadIdx.x == 0) {
+    state[0] = ~0ull;
+    state[1] = 0ull;
+  }
+}
+
+bool pow_grind_dev(int engine, const uint8_t challenge[32], uint64_t threshold, uint64_t first, uint64_t count,
+                   uint64_t* nonce, hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, 64), "pow: engine does not hash 64-byte messages");
+  WHIR_REQUIRE(count > 0 && count % POW_CHUNK == 0 && first % POW_CHUNK == 0,
+               "pow: window must be whole chunks");
+  uint32_t w[8];
+  for (int i = 0; i < 8; i++)
+    w[i] = (uint32_t)challenge[4 * i] | ((uint32_t)challenge[4 * i + 1] << 8) |
+           ((uint32_t)challenge[4 * i + 2] << 16) | ((uint32_t)challenge[4 * i + 3] << 24);
+  DevBuf state(16, st);
+  k_pow_init<<<1, 64, 0, st>>>(state.as<unsigned long long>());
+  WHIR_HIP(hipGetLastError());
+  int dev = 0, cus = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  WHIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint64_t num_chunks = count / POW_CHUNK;
+  unsigned blocks = (unsigned)std::min<uint64_t>((uint64_t)cus * 8, num_chunks);
+  k_pow<<<blocks, POW_THREADS, 0, st>>>(engine, make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]),
+                                        threshold, first, num_chunks, state.as<unsigned long long>());
+  WHIR_HIP(hipGetLastError());
+  unsigned long long host = 0;
+  WHIR_HIP(hipMemcpyAsync(&host, state.ptr, 8, hipMemcpyDeviceToHost, st));
+  WHIR_HIP(hipStreamSynchronize(st));
+  if (host == ~0ull) return false;
+  *nonce = host;
+  return true;
+}
+
+uint64_t pow_chunk() { return POW_CHUNK; }
+
+}  // namespace whir
