@@ -189,6 +189,77 @@ int whir_pow_grind(int engine, const uint8_t challenge[32], uint64_t threshold, 
 int whir_pow_grind_bounded(int engine, const uint8_t challenge[32], uint64_t threshold, uint64_t max_nonces,
                            uint64_t *nonce, int *found);
 
+/* ------------------------------------------------------------ WHIR prover
+ * Replaces whir::Config::new (src/protocols/whir/config.rs:16-152),
+ * whir::Config::commit (src/protocols/whir/mod.rs:79-92 -> irs_commit.rs:292-350)
+ * and whir::Config::prove (src/protocols/whir/prover.rs:50-316) for one committed
+ * base-field vector (batch_size 1, Basefield<Field64_D> embedding, or Identity
+ * for degree 1) and MultilinearExtension linear forms (evaluation points). */
+
+/* ProtocolParameters (src/parameters.rs:8-24) plus the field degree. */
+typedef struct whir_params {
+  int degree;                 /* 1, 2 or 3: Goldilocks1/2/3 (src/cmdline_utils.rs:7-15) */
+  int security_level;
+  int pow_bits;
+  int initial_folding_factor;
+  int folding_factor;
+  int unique_decoding;
+  int starting_log_inv_rate;
+  int batch_size;             /* must be 1 */
+  int hash_engine;            /* WHIR_HASH_* */
+} whir_params;
+
+typedef struct whir_config whir_config;
+typedef struct whir_transcript whir_transcript;
+typedef struct whir_witness whir_witness;
+
+/* Config::new for a vector of `size` (a power of two) coefficients. */
+int whir_config_new(const whir_params *params, size_t size, whir_config **out);
+/* Sets every PoW threshold to u64::MAX (the reference's disable_pow, whir/mod.rs:107-118). */
+int whir_config_disable_pow(whir_config *cfg);
+/* JSON description of the derived round shapes, samples and PoW thresholds. */
+int whir_config_describe(const whir_config *cfg, char *buf, size_t cap, size_t *len);
+void whir_config_free(whir_config *cfg);
+
+/* Typed transcript operations (ProverState, src/transcript/mod.rs:183-267); each
+ * callback returns 0 on success.  A Rust binding implements them over spongefish. */
+typedef struct whir_transcript_vtable {
+  void *ctx;
+  int (*prover_field)(void *ctx, int degree, const uint64_t *elems, size_t count);
+  int (*verifier_field)(void *ctx, int degree, uint64_t *out);
+  int (*verifier_bytes)(void *ctx, uint8_t *out, size_t count);
+  int (*prover_hash)(void *ctx, const uint8_t *hash32);
+  int (*prover_u64)(void *ctx, uint64_t value);
+  int (*hint_hash)(void *ctx, const uint8_t *hash32);
+  int (*hint_field_vec)(void *ctx, int degree, const uint64_t *elems, size_t count);
+} whir_transcript_vtable;
+
+/* Built-in transcript: Keccak-f[1600] duplex keyed by a 64-byte protocol id and a
+ * 32-byte session id (DomainSeparator, src/transcript/mod.rs:110-137). */
+int whir_transcript_new(const uint8_t protocol_id[64], const uint8_t session_id[32], const uint8_t *instance,
+                        size_t instance_len, whir_transcript **out);
+int whir_transcript_from_vtable(const whir_transcript_vtable *vtable, whir_transcript **out);
+/* Proof bytes of a built-in transcript (Proof{narg_string, hints}, transcript/mod.rs:63-70).
+ * Pass NULL buffers to query the lengths. */
+int whir_transcript_proof(const whir_transcript *t, uint8_t *narg, size_t narg_cap, size_t *narg_len,
+                          uint8_t *hints, size_t hints_cap, size_t *hints_len);
+void whir_transcript_free(whir_transcript *t);
+
+/* whir::Config::commit of a device-resident base-field vector of `size` words.
+ * The witness (codeword and Merkle tree) stays on the device. */
+int whir_commit_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *vector, whir_witness **out);
+/* whir::Config::prove: `points` holds num_points points of num_variables
+ * elements (degree words each), `evaluations` the claimed MLE values.
+ * evaluation_point (optional) receives the FinalClaim point (num_variables elements). */
+int whir_prove_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *vector, const whir_witness *w,
+                   const uint64_t *points, size_t num_points, const uint64_t *evaluations,
+                   uint64_t *evaluation_point);
+void whir_witness_free(whir_witness *w);
+/* MultilinearExtension::evaluate (mixed_multilinear_extend, src/algebra/multilinear.rs:15-151)
+ * of a device base-field vector of 2^num_variables words at a degree-`degree` point. */
+int whir_mle_evaluate_dev(int degree, const uint64_t *vector, size_t num_variables, const uint64_t *point,
+                          uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

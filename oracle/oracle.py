@@ -104,6 +104,8 @@ def _declare(L):
     L.oracle_dot.argtypes = [ci, u64p, u64p, sz, u64p]
     L.oracle_pow_search.argtypes = [ci, u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u64p]
     L.oracle_bytes_to_indices.argtypes = [u8p, sz, sz, szp]
+    L.oracle_pow_threshold.argtypes = [ctypes.c_double]
+    L.oracle_pow_threshold.restype = ctypes.c_uint64
 
 
 def _p64(a: np.ndarray):
@@ -381,13 +383,10 @@ def dot(D: int, a, b) -> np.ndarray:
 
 # ---------------------------------------------------------------- PoW / indices
 def pow_threshold(bits: float) -> int:
-    """proof_of_work::threshold, src/protocols/proof_of_work.rs:29-39."""
-    import math
+    """proof_of_work::threshold, src/protocols/proof_of_work.rs:29-39 (libm exp2, as the
+    reference's f64::exp2)."""
     assert 0.0 <= bits <= 60.0
-    t = math.ceil(2.0 ** (64.0 - bits))
-    if t >= float(2**64 - 1):
-        return 2**64 - 1
-    return int(t)
+    return int(lib().oracle_pow_threshold(float(bits)))
 
 
 def pow_search(engine: int, challenge: bytes, threshold: int, start: int = 0, limit: int = 1 << 40):

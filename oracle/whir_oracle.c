@@ -959,3 +959,11 @@ void oracle_bytes_to_indices(const u8 *entropy, size_t count, size_t num_leaves,
         out[i] = acc % num_leaves;
     }
 }
+
+/* proof_of_work::threshold, src/protocols/proof_of_work.rs:29-39 (f64 exp2, ceil). */
+#include <math.h>
+u64 oracle_pow_threshold(double bits) {
+    double t = ceil(exp2(64.0 - bits));
+    if (t >= 18446744073709551615.0) return ~0ULL;
+    return (u64)t;
+}

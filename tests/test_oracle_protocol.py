@@ -1,0 +1,153 @@
+"""Pins the oracle's protocol layer (oracle/whir_protocol.py) on the CPU.
+
+- parameter selection against the round shapes of the three headline configs
+  (an independent restatement of src/protocols/whir/config.rs:16-152 and
+  src/protocols/irs_commit.rs:125-288 tabulated with the survey of the reference);
+- prove -> verify round trips over fields, hashes, folding factors, rates and
+  PoW settings (the reference's make_whir_things matrix, whir/mod.rs:176-894);
+- negative tests: tampered proofs are rejected (whir/mod.rs:515-612 analogue).
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import whir_protocol as WP
+
+
+def shapes(cfg):
+    out = [(cfg.initial.message_length(), cfg.initial.codeword_length, cfg.initial.out_domain_samples,
+            cfg.initial.in_domain_samples)]
+    for r in cfg.rounds:
+        out.append((r.irs.message_length(), r.irs.codeword_length, r.irs.out_domain_samples,
+                    r.irs.in_domain_samples))
+    return out
+
+
+def bits(thr):
+    return round(WP.pow_difficulty(thr), 2)
+
+
+def test_config_c1_shapes():
+    p = WP.ProtocolParameters(security_level=100, pow_bits=20, starting_log_inv_rate=1)
+    cfg = WP.WhirConfig.new(1 << 20, p, 2)
+    assert shapes(cfg) == [(1 << 16, 1 << 17, 1, 187), (1 << 12, 1 << 16, 1, 42), (1 << 8, 1 << 15, 1, 24),
+                           (1 << 4, 1 << 14, 1, 17), (1, 1 << 13, 1, 13)]
+    assert bits(cfg.initial_sumcheck.round_pow) == 30.75
+    assert [bits(r.pow) for r in cfg.rounds] == [19.66, 18.96, 17.69, 16.2]
+    assert [bits(r.sumcheck.round_pow) for r in cfg.rounds] == [33.25, 35.75, 38.25, 40.75]
+    assert bits(cfg.final_pow) == 16.42
+    assert cfg.final_sumcheck.initial_size == 1 and cfg.final_sumcheck.num_rounds == 0
+
+
+def test_config_c2_shapes():
+    p = WP.ProtocolParameters(security_level=100, pow_bits=20, starting_log_inv_rate=2)
+    cfg = WP.WhirConfig.new(1 << 24, p, 2)
+    assert shapes(cfg) == [(1 << 20, 1 << 22, 1, 87), (1 << 16, 1 << 21, 1, 33), (1 << 12, 1 << 20, 1, 21),
+                           (1 << 8, 1 << 19, 1, 15), (1 << 4, 1 << 18, 1, 12), (1, 1 << 17, 1, 10)]
+    assert bits(cfg.initial_sumcheck.round_pow) == 42.25
+    assert [bits(r.pow) for r in cfg.rounds] == [19.12, 19.82, 17.48, 18.56, 16.84]
+    assert [bits(r.sumcheck.round_pow) for r in cfg.rounds] == [44.75, 47.25, 49.75, 52.25, 54.75]
+    assert bits(cfg.final_pow) == 15.7
+
+
+def test_config_c3_shapes():
+    p = WP.ProtocolParameters(security_level=100, pow_bits=20, starting_log_inv_rate=1, hash_engine=O.ENGINE_SHA3)
+    cfg = WP.WhirConfig.new(1 << 26, p, 3)
+    assert shapes(cfg) == [(1 << 22, 1 << 23, 1, 187), (1 << 18, 1 << 22, 1, 42), (1 << 14, 1 << 21, 1, 24),
+                           (1 << 10, 1 << 20, 1, 17), (1 << 6, 1 << 19, 1, 13), (1 << 2, 1 << 18, 1, 11)]
+    assert cfg.initial_sumcheck.round_pow == 2**64 - 1
+    assert [bits(r.pow) for r in cfg.rounds] == [19.66, 18.96, 17.69, 16.2, 16.42]
+    assert bits(cfg.final_pow) == 12.77
+    assert cfg.final_sumcheck.initial_size == 4
+
+
+def test_config_unique_decoding_has_no_ood():
+    p = WP.ProtocolParameters(security_level=32, pow_bits=0, unique_decoding=True)
+    cfg = WP.WhirConfig.new(1 << 12, p, 3)
+    assert cfg.initial.out_domain_samples == 0
+    assert all(r.irs.out_domain_samples == 0 for r in cfg.rounds)
+
+
+def test_transcript_is_deterministic_and_order_sensitive():
+    a = WP.Transcript(b"p" * 64, b"s" * 32)
+    b = WP.Transcript(b"p" * 64, b"s" * 32)
+    a.prover_hash(b"\1" * 32)
+    b.prover_hash(b"\1" * 32)
+    assert a.verifier_bytes(40) == b.verifier_bytes(40)
+    c = WP.Transcript(b"p" * 64, b"s" * 32)
+    c.prover_hash(b"\2" * 32)
+    d = WP.Transcript(b"p" * 64, b"s" * 32)
+    d.prover_hash(b"\1" * 32)
+    assert c.verifier_bytes(8) != d.verifier_bytes(8)
+    # field challenges are canonical
+    e = WP.Transcript(b"q" * 64, b"s" * 32)
+    assert all(int(x) < O.P for _ in range(50) for x in e.verifier_field(3))
+
+
+def rand_vec(seed, n):
+    g = np.random.default_rng(seed)
+    v = g.integers(0, O.P, size=n, dtype=np.uint64)
+    return v
+
+
+def run_roundtrip(nv, D, engine, k0, k, lir, num_points, ud=False, sec=32, pow_bits=0, disable_pow=True,
+                  seed=1):
+    p = WP.ProtocolParameters(security_level=sec, pow_bits=pow_bits, initial_folding_factor=k0,
+                              folding_factor=k, unique_decoding=ud, starting_log_inv_rate=lir,
+                              hash_engine=engine)
+    cfg = WP.WhirConfig.new(1 << nv, p, D)
+    if disable_pow:
+        cfg.disable_pow()
+    pid, sid = WP.protocol_ids(f"test nv={nv} D={D} e={engine} k0={k0} k={k} lir={lir}")
+    vec = rand_vec(seed, 1 << nv)
+    rng = random.Random(seed)
+    points = [[np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)]
+              for _ in range(num_points)]
+    evals = [WP.mle_evaluate_base(D, vec, pt) for pt in points]
+    t = WP.Transcript(pid, sid)
+    w = WP.commit(t, cfg, vec)
+    WP.prove(t, cfg, vec, w, points, evals)
+    r = WP.TranscriptReader(pid, sid, bytes(t.narg), bytes(t.hints))
+    WP.verify(r, cfg, evals, points)
+    return cfg, pid, sid, bytes(t.narg), bytes(t.hints), evals, points
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("num_points", [0, 1, 2])
+def test_prove_verify_roundtrip_fields(D, num_points):
+    run_roundtrip(10, D, O.ENGINE_BLAKE3, 4, 4, 1, num_points)
+
+
+@pytest.mark.parametrize("k0,k", [(1, 1), (2, 3), (3, 2), (4, 4)])
+@pytest.mark.parametrize("engine", [O.ENGINE_SHA3, O.ENGINE_BLAKE3])
+def test_prove_verify_roundtrip_folding(k0, k, engine):
+    run_roundtrip(9, 2, engine, k0, k, 2, 1)
+
+
+def test_prove_verify_unique_decoding_and_pow():
+    run_roundtrip(8, 3, O.ENGINE_SHA3, 2, 2, 1, 1, ud=True, sec=24, pow_bits=6, disable_pow=False)
+    run_roundtrip(8, 2, O.ENGINE_BLAKE3, 4, 4, 1, 2, ud=False, sec=40, pow_bits=10, disable_pow=False)
+
+
+def test_tampered_proofs_are_rejected():
+    cfg, pid, sid, narg, hints, evals, points = run_roundtrip(8, 2, O.ENGINE_BLAKE3, 2, 2, 1, 1)
+    rng = random.Random(9)
+    for _ in range(6):
+        pos = rng.randrange(len(narg))
+        bad = bytearray(narg)
+        bad[pos] ^= 1 << rng.randrange(8)
+        with pytest.raises((WP.VerificationError, AssertionError, ValueError)):
+            WP.verify(WP.TranscriptReader(pid, sid, bytes(bad), hints), cfg, evals, points)
+    for _ in range(6):
+        pos = rng.randrange(len(hints))
+        bad = bytearray(hints)
+        bad[pos] ^= 1 << rng.randrange(8)
+        with pytest.raises((WP.VerificationError, AssertionError, ValueError)):
+            WP.verify(WP.TranscriptReader(pid, sid, narg, bytes(bad)), cfg, evals, points)
+    # a wrong claimed evaluation
+    wrong = [WP.e_add(2, evals[0], WP.e_one(2))]
+    with pytest.raises(WP.VerificationError):
+        WP.verify(WP.TranscriptReader(pid, sid, narg, hints), cfg, wrong, points)

@@ -96,7 +96,21 @@ SIGNATURES = {
     "whir_pow_threshold": [ctypes.c_double, u64p],
     "whir_pow_grind": [ci, u8p, u64, u64p, ctypes.POINTER(ci)],
     "whir_pow_grind_bounded": [ci, u8p, u64, u64, u64p, ctypes.POINTER(ci)],
+    # prover (whir_amd/prover.py)
+    "whir_config_new": [vp, sz, ctypes.POINTER(vp)],
+    "whir_config_disable_pow": [vp],
+    "whir_config_describe": [vp, ctypes.c_char_p, sz, szp],
+    "whir_config_free": [vp],
+    "whir_transcript_new": [u8p, u8p, u8p, sz, ctypes.POINTER(vp)],
+    "whir_transcript_from_vtable": [vp, ctypes.POINTER(vp)],
+    "whir_transcript_proof": [vp, u8p, sz, szp, u8p, sz, szp],
+    "whir_transcript_free": [vp],
+    "whir_commit_dev": [vp, vp, vp, ctypes.POINTER(vp)],
+    "whir_prove_dev": [vp, vp, vp, vp, u64p, sz, u64p, u64p],
+    "whir_witness_free": [vp],
+    "whir_mle_evaluate_dev": [ci, vp, sz, u64p, u64p],
 }
+VOID_FUNCS = {"whir_config_free", "whir_transcript_free", "whir_witness_free"}
 
 
 def lib():
@@ -109,7 +123,7 @@ def lib():
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = ci
+            fn.restype = None if name in VOID_FUNCS else ci
         L.whir_last_error.argtypes = []
         L.whir_last_error.restype = ctypes.c_char_p
         _lib = L

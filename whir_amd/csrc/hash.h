@@ -26,6 +26,8 @@ void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, 
 // Merkle nodes over the first num_leaves leaf hashes already stored in
 // nodes[0 .. num_leaves); nodes holds (2^(layers+1) - 1) hashes.
 size_t merkle_layers(size_t num_leaves);
+// out[k] = nodes[idx[k]] (32-byte hashes), idx a device array of k node indices.
+void gather_hashes_dev(const uint8_t* nodes, const uint64_t* idx, size_t k, uint8_t* out, hipStream_t st);
 size_t merkle_num_nodes(size_t num_leaves);
 void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st);
 

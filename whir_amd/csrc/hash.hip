@@ -140,6 +140,20 @@ size_t merkle_layers(size_t n) {
 }
 size_t merkle_num_nodes(size_t n) { return ((size_t)2 << merkle_layers(n)) - 1; }
 
+__global__ void k_gather_hashes(const uint4* __restrict__ nodes, const uint64_t* __restrict__ idx, size_t k,
+                                uint4* __restrict__ out) {
+  size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= 2 * k) return;
+  size_t q = e >> 1, half = e & 1;
+  out[e] = nodes[2 * idx[q] + half];
+}
+
+void gather_hashes_dev(const uint8_t* nodes, const uint64_t* idx, size_t k, uint8_t* out, hipStream_t st) {
+  if (!k) return;
+  k_gather_hashes<<<blocks_for(2 * k, 256), 256, 0, st>>>((const uint4*)nodes, idx, k, (uint4*)out);
+  WHIR_HIP(hipGetLastError());
+}
+
 static constexpr int MERKLE_THREADS = 256;
 
 // Input level of width `w` at nodes + in_off*32; reduces `levels` levels.  Block b

@@ -1,0 +1,546 @@
+// WHIR prover orchestration on the GPU (host control, device data).
+//
+// Restates the transcript-ordered control flow of the reference prover:
+//   commit   whir::Config::commit -> irs_commit::Config::commit
+//            (src/protocols/whir/mod.rs:79-92, src/protocols/irs_commit.rs:292-350)
+//   prove    whir::Config::prove (src/protocols/whir/prover.rs:50-316)
+//   open     irs_commit::Config::open (irs_commit.rs:385-432), merkle_tree::Config::open
+//            (src/protocols/merkle_tree.rs:145-186), challenge_indices
+//            (src/protocols/challenge_indices.rs:6-50)
+//   sumcheck sumcheck::Config::prove without masks (src/protocols/sumcheck.rs:69-162)
+//   pow      proof_of_work::Config::prove (src/protocols/proof_of_work.rs:68-166)
+// with one committed base-field vector and MultilinearExtension linear forms.
+//
+// Every vector of the protocol (lifted vector, covector, codewords, Merkle trees)
+// lives in HBM for the whole proof; the host only sees the transcript values:
+// roots, OOD and sumcheck messages, opened rows and sibling hashes.
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "../../include/whir_amd.h"
+#include "algebra.h"
+#include "common.h"
+#include "config.h"
+#include "gl.h"
+#include "hash.h"
+#include "ntt.h"
+#include "transcript.h"
+
+using namespace whir;
+
+struct whir_config {
+  WhirCfg cfg;
+};
+
+struct whir_transcript {
+  std::unique_ptr<Transcript> t;
+  Duplex* duplex = nullptr;  // set for the built-in transcript
+};
+
+namespace whir {
+
+struct DevWitness {
+  int Ds = 1;            // degree of the committed elements
+  size_t N = 0, M = 0;   // codeword length, number of messages (columns)
+  DevBuf cw;             // column-major codeword cw[j*N + r]
+  DevBuf nodes;          // Merkle nodes [padded leaves][layer]...[root]
+  std::vector<uint64_t> ood_points, ood_values;  // D words each
+};
+
+}  // namespace whir
+
+struct whir_witness {
+  DevWitness w;
+};
+
+namespace whir {
+namespace {
+
+template <int D>
+using E = gl::Ext<D>;
+
+template <int D>
+E<D> ld(const uint64_t* p) {
+  E<D> r;
+  for (int d = 0; d < D; d++) r.c[d] = p[d];
+  return r;
+}
+template <int D>
+void push(std::vector<uint64_t>& v, const E<D>& e) {
+  for (int d = 0; d < D; d++) v.push_back(e.c[d]);
+}
+
+template <int D>
+E<D> verifier_field(Transcript& t) {
+  E<D> r;
+  t.verifier_field(D, r.c);
+  return r;
+}
+
+// geometric_challenge, src/protocols/geometric_challenge.rs:10-24
+template <int D>
+std::vector<E<D>> geometric_challenge(Transcript& t, size_t count) {
+  std::vector<E<D>> out;
+  if (count == 0) return out;
+  if (count == 1) {
+    out.push_back(gl::ext_base<D>(1));
+    return out;
+  }
+  E<D> x = verifier_field<D>(t);
+  E<D> cur = gl::ext_base<D>(1);
+  for (size_t i = 0; i < count; i++) {
+    out.push_back(cur);
+    cur = gl::emul<D>(cur, x);
+  }
+  return out;
+}
+
+// challenge_indices, src/protocols/challenge_indices.rs:6-50
+std::vector<size_t> challenge_indices(Transcript& t, size_t num_leaves, size_t count, bool dedup) {
+  std::vector<size_t> idx;
+  if (count == 0) return idx;
+  WHIR_REQUIRE(num_leaves && (num_leaves & (num_leaves - 1)) == 0,
+               "Number of leaves must be a power of two for unbiased results.");
+  if (num_leaves == 1) return dedup ? std::vector<size_t>{0} : std::vector<size_t>(count, 0);
+  size_t lg = 0;
+  while (((size_t)1 << lg) < num_leaves) lg++;
+  size_t size_bytes = (lg + 7) / 8;
+  std::vector<uint8_t> entropy(count * size_bytes);
+  for (auto& b : entropy) t.verifier_bytes(&b, 1);
+  for (size_t i = 0; i < count; i++) {
+    size_t acc = 0;
+    for (size_t k = 0; k < size_bytes; k++) acc = (acc << 8) | entropy[i * size_bytes + k];
+    idx.push_back(acc % num_leaves);
+  }
+  if (dedup) {
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+  }
+  return idx;
+}
+
+// proof_of_work::Config::prove, src/protocols/proof_of_work.rs:68-166
+void pow_prove(Transcript& t, int engine, uint64_t threshold, hipStream_t s) {
+  if (threshold == UINT64_MAX) return;
+  uint8_t ch[32];
+  t.verifier_bytes(ch, 32);
+  const uint64_t window = 1ull << 32;
+  uint64_t nonce = 0;
+  for (uint64_t first = 0;; first += window) {
+    if (pow_grind_dev(engine, ch, threshold, first, window, &nonce, s)) break;
+    WHIR_REQUIRE(first <= UINT64_MAX - 2 * window, "Proof of Work failed to solve.");
+  }
+  t.prover_u64(nonce);
+}
+
+int log2_exact(uint64_t n) {
+  int k = 0;
+  while ((1ull << k) < n) k++;
+  return (1ull << k) == n ? k : -1;
+}
+
+uint64_t coset_size(uint64_t masked_len, uint64_t N) {
+  // cooley_tukey.rs:436-439 restricted to the power-of-two codewords WHIR uses
+  uint64_t c = 1;
+  while (c < masked_len) c <<= 1;
+  WHIR_REQUIRE(N % c == 0, "coset size must divide the codeword length");
+  return c;
+}
+
+template <int D>
+void d2h(E<D>& out, const uint64_t* dev, hipStream_t s) {
+  WHIR_HIP(hipMemcpyAsync(out.c, dev, D * 8, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+}
+
+// irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (one vector,
+// no masks).  vec holds vector_size elements of Ds words on the device.
+template <int D>
+void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec, int Ds, DevWitness& w,
+                hipStream_t s) {
+  WHIR_REQUIRE(c.num_vectors == 1, "irs commit: one vector per commitment");
+  const size_t M = c.num_messages(), L = c.message_length(), N = c.codeword_length;
+  uint64_t C = coset_size(L, N);
+  w.Ds = Ds;
+  w.N = N;
+  w.M = M;
+  w.cw = DevBuf(M * N * Ds * 8, s);
+  RsArgs a{vec, L, L, nullptr, 0, w.cw.as<uint64_t>(), N, N / C, log2_exact(C), M};
+  {
+    DevBuf scratch(rs_scratch_words(Ds, M, N, C) * 8, s);
+    rs_encode_cols(Ds, a, scratch.as<uint64_t>(), s);
+  }
+  size_t nn = merkle_num_nodes(N);
+  w.nodes = DevBuf(32 * nn, s);
+  hash_rows_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s);
+  merkle_build_dev(engine, w.nodes.as<uint8_t>(), N, s);
+  uint8_t root[32];
+  WHIR_HIP(hipMemcpyAsync(root, w.nodes.as<uint8_t>() + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+  t.prover_hash(root);
+  w.ood_points.clear();
+  w.ood_values.clear();
+  for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
+  DevBuf out(D * 8, s);
+  for (size_t i = 0; i < c.out_domain_samples; i++) {
+    univariate_eval_dev(Ds, D, vec, c.vector_size, &w.ood_points[i * D], out.as<uint64_t>(), s);
+    E<D> v;
+    d2h<D>(v, out.as<uint64_t>(), s);
+    t.prover_field(D, v.c, 1);
+    push<D>(w.ood_values, v);
+  }
+}
+
+// irs_commit::Config::open for one witness: hints the opened rows and their
+// Merkle paths; returns evaluation points (base field) and rows (Ds words).
+template <int D>
+void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<uint64_t>& pts,
+              std::vector<uint64_t>& rows, hipStream_t s) {
+  std::vector<size_t> idx = challenge_indices(t, c.codeword_length, c.in_domain_samples, c.deduplicate_in_domain);
+  const size_t N = w.N, M = w.M, k = idx.size();
+  uint64_t C = coset_size(c.masked_message_length(), N), Q = N / C;
+  uint64_t g = gl::root_of_unity(N);
+  pts.clear();
+  for (size_t i : idx) pts.push_back(gl::pow(g, (i / C) + (i % C) * Q));  // transpose_permute
+  rows.assign(k * M * w.Ds, 0);
+  if (k) {
+    std::vector<uint64_t> idx64(idx.begin(), idx.end());
+    DevBuf didx(k * 8, s), drows(k * M * w.Ds * 8, s);
+    WHIR_HIP(hipMemcpyAsync(didx.ptr, idx64.data(), k * 8, hipMemcpyHostToDevice, s));
+    gather_rows_dev(w.Ds, w.cw.as<uint64_t>(), N, M, didx.as<uint64_t>(), k, drows.as<uint64_t>(), s);
+    WHIR_HIP(hipMemcpyAsync(rows.data(), drows.ptr, k * M * w.Ds * 8, hipMemcpyDeviceToHost, s));
+    WHIR_HIP(hipStreamSynchronize(s));
+  }
+  t.hint_field_vec(w.Ds, rows.data(), k * M);
+  // merkle_tree::Config::open: sibling node indices in hint order
+  std::vector<size_t> cur(idx);
+  std::sort(cur.begin(), cur.end());
+  cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+  std::vector<uint64_t> sib;
+  size_t layers = merkle_layers(N), off = 0, width = (size_t)1 << layers;
+  while (width > 1) {
+    std::vector<size_t> next;
+    for (size_t q = 0; q < cur.size();) {
+      size_t a = cur[q];
+      if (q + 1 < cur.size() && cur[q + 1] == (a ^ 1)) {
+        q += 2;
+      } else {
+        sib.push_back(off + (a ^ 1));
+        q += 1;
+      }
+      next.push_back(a >> 1);
+    }
+    cur.swap(next);
+    off += width;
+    width >>= 1;
+  }
+  if (!sib.empty()) {
+    std::vector<uint8_t> hashes(32 * sib.size());
+    DevBuf dsib(sib.size() * 8, s), dh(32 * sib.size(), s);
+    WHIR_HIP(hipMemcpyAsync(dsib.ptr, sib.data(), sib.size() * 8, hipMemcpyHostToDevice, s));
+    gather_hashes_dev(w.nodes.as<uint8_t>(), dsib.as<uint64_t>(), sib.size(), dh.as<uint8_t>(), s);
+    WHIR_HIP(hipMemcpyAsync(hashes.data(), dh.ptr, hashes.size(), hipMemcpyDeviceToHost, s));
+    WHIR_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < sib.size(); i++) t.hint_hash(&hashes[32 * i]);
+  }
+}
+
+// sumcheck::Config::prove (no masks), staggered fold + round polynomial.
+template <int D>
+std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, uint64_t* a, uint64_t* b,
+                                 size_t& n, E<D>& sum, hipStream_t s) {
+  WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
+  std::vector<E<D>> chals;
+  DevBuf dout(2 * D * 8, s);
+  bool have_prev = false;
+  E<D> prev;
+  for (size_t round = 0; round < sc.num_rounds; round++) {
+    if (have_prev) n = fold_and_poly_dev(D, a, b, n, prev.c, dout.as<uint64_t>(), s);
+    else sumcheck_poly_dev(D, a, b, n, dout.as<uint64_t>(), s);
+    uint64_t h[2 * D];
+    WHIR_HIP(hipMemcpyAsync(h, dout.ptr, 2 * D * 8, hipMemcpyDeviceToHost, s));
+    WHIR_HIP(hipStreamSynchronize(s));
+    E<D> c0 = ld<D>(h), c2 = ld<D>(h + D);
+    E<D> c1 = gl::esub<D>(gl::esub<D>(gl::esub<D>(sum, c0), c0), c2);
+    t.prover_field(D, c0.c, 1);
+    t.prover_field(D, c2.c, 1);
+    pow_prove(t, engine, sc.round_pow, s);
+    E<D> r = verifier_field<D>(t);
+    chals.push_back(r);
+    sum = gl::eadd<D>(gl::emul<D>(gl::eadd<D>(gl::emul<D>(c2, r), c1), r), c0);
+    prev = r;
+    have_prev = true;
+  }
+  if (have_prev) {
+    size_t m = fold_dev(D, a, n, prev.c, s);
+    fold_dev(D, b, n, prev.c, s);
+    n = m;
+  }
+  return chals;
+}
+
+// eq_weights (src/algebra/multilinear.rs:154-158) on the host: 2^k weights, x_0 = MSB.
+template <int D>
+std::vector<E<D>> eq_weights(const std::vector<E<D>>& z) {
+  std::vector<E<D>> cur{gl::ext_base<D>(1)};
+  for (const auto& zi : z) {
+    E<D> om = gl::esub<D>(gl::ext_base<D>(1), zi);
+    std::vector<E<D>> nxt(cur.size() * 2);
+    for (size_t x = 0; x < cur.size(); x++) {
+      nxt[2 * x] = gl::emul<D>(cur[x], om);
+      nxt[2 * x + 1] = gl::emul<D>(cur[x], zi);
+    }
+    cur.swap(nxt);
+  }
+  return cur;
+}
+
+template <int D>
+void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* vec, DevWitness& w, hipStream_t s) {
+  irs_commit<D>(t, cfg.initial, cfg.engine, vec, 1, w, s);
+}
+
+// whir::Config::prove, src/protocols/whir/prover.rs:50-316.
+template <int D>
+void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, const DevWitness& w0,
+                const uint64_t* points, size_t np, const uint64_t* evals, uint64_t* eval_point_out, hipStream_t s) {
+  const int eng = cfg.engine;
+  size_t n = cfg.initial.vector_size;
+  size_t nv = 0;
+  while (((size_t)1 << nv) < n) nv++;
+  // Random linear combination of the vectors (one vector: [1], no transcript).
+  std::vector<E<D>> vector_rlc = geometric_challenge<D>(t, 1);
+  DevBuf vec(n * D * 8, s), cov(n * D * 8, s);
+  lift_dev(D, dvector, n, vec.as<uint64_t>(), s);
+  const size_t nood = w0.ood_points.size() / D;
+  std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
+  const bool has_constraints = !crlc.empty();
+  fill_zero(cov.ptr, n * D * 8, s);
+  for (size_t k = 0; k < np; k++) eval_eq_dev(D, cov.as<uint64_t>(), points + k * nv * D, nv, crlc[k].c, s);
+  E<D> the_sum = gl::ext_zero<D>();
+  for (size_t k = 0; k < np; k++)
+    the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[k], gl::emul<D>(vector_rlc[0], ld<D>(evals + k * D))));
+  if (nood) {
+    std::vector<uint64_t> srlc;
+    for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
+    geometric_accumulate_dev(D, cov.as<uint64_t>(), n, srlc.data(), w0.ood_points.data(), nood, s);
+  }
+  for (size_t k = 0; k < nood; k++)
+    the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[np + k],
+                                               gl::emul<D>(vector_rlc[0], ld<D>(&w0.ood_values[k * D]))));
+  std::vector<E<D>> fr;
+  if (has_constraints) {
+    fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+  } else {
+    for (size_t i = 0; i < cfg.initial_sumcheck.num_rounds; i++) fr.push_back(verifier_field<D>(t));
+    pow_prove(t, eng, cfg.initial_skip_pow, s);
+    for (const auto& f : fr) n = fold_dev(D, vec.as<uint64_t>(), n, f.c, s);
+    fill_zero(cov.ptr, n * D * 8, s);  // the covector is all zeros
+  }
+  std::vector<E<D>> eval_point(fr);
+
+  const IrsCfg* prev_cfg = &cfg.initial;
+  const DevWitness* prev_w = &w0;
+  std::unique_ptr<DevWitness> held;
+  for (const RoundCfg& rc : cfg.rounds) {
+    auto nw = std::make_unique<DevWitness>();
+    irs_commit<D>(t, rc.irs, eng, vec.as<uint64_t>(), D, *nw, s);
+    pow_prove(t, eng, rc.pow, s);
+    std::vector<uint64_t> pts, rows;
+    irs_open<D>(t, *prev_cfg, *prev_w, pts, rows, s);
+    // STIR constraints: OOD points of the new commitment, then the opened rows.
+    std::vector<E<D>> w_eq = eq_weights<D>(fr);  // tensor_product([1], eq_weights(r))
+    std::vector<uint64_t> spoints(nw->ood_points);
+    std::vector<E<D>> svals;
+    for (size_t k = 0; k < nw->ood_values.size() / D; k++) svals.push_back(ld<D>(&nw->ood_values[k * D]));
+    const size_t M = prev_w->M;
+    const int Ds = prev_w->Ds;
+    for (size_t q = 0; q < pts.size(); q++) {
+      push<D>(spoints, gl::ext_base<D>(pts[q]));
+      E<D> acc = gl::ext_zero<D>();
+      for (size_t j = 0; j < M; j++) {
+        const uint64_t* e = &rows[(q * M + j) * Ds];
+        E<D> v = Ds == 1 ? gl::ext_base<D>(e[0]) : ld<D>(e);
+        acc = gl::eadd<D>(acc, gl::emul<D>(w_eq[j], v));
+      }
+      svals.push_back(acc);
+    }
+    std::vector<E<D>> srlc = geometric_challenge<D>(t, svals.size());
+    if (!srlc.empty()) {
+      std::vector<uint64_t> flat;
+      for (auto& e : srlc) push<D>(flat, e);
+      geometric_accumulate_dev(D, cov.as<uint64_t>(), n, flat.data(), spoints.data(), srlc.size(), s);
+    }
+    for (size_t k = 0; k < srlc.size(); k++) the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], svals[k]));
+    fr = sumcheck_prove<D>(t, rc.sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+    eval_point.insert(eval_point.end(), fr.begin(), fr.end());
+    held = std::move(nw);
+    prev_cfg = &rc.irs;
+    prev_w = held.get();
+  }
+  // Send the final vector directly.
+  WHIR_REQUIRE(n == cfg.final_sumcheck.initial_size, "final vector length mismatch");
+  std::vector<uint64_t> fin(n * D);
+  WHIR_HIP(hipMemcpyAsync(fin.data(), vec.ptr, n * D * 8, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++) t.prover_field(D, &fin[i * D], 1);
+  pow_prove(t, eng, cfg.final_pow, s);
+  {
+    std::vector<uint64_t> pts, rows;
+    irs_open<D>(t, *prev_cfg, *prev_w, pts, rows, s);
+  }
+  fr = sumcheck_prove<D>(t, cfg.final_sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+  eval_point.insert(eval_point.end(), fr.begin(), fr.end());
+  if (eval_point_out)
+    for (size_t i = 0; i < eval_point.size(); i++)
+      for (int d = 0; d < D; d++) eval_point_out[i * D + d] = eval_point[i].c[d];
+  WHIR_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+}  // namespace whir
+
+#define API_BEGIN try {
+#define API_END                                \
+  }                                            \
+  catch (const std::exception& e) {            \
+    set_last_error(e.what());                  \
+    return 1;                                  \
+  }                                            \
+  catch (...) {                                \
+    set_last_error("unknown error");           \
+    return 1;                                  \
+  }                                            \
+  set_last_error("");                          \
+  return 0;
+
+extern "C" {
+
+int whir_config_new(const whir_params* p, size_t size, whir_config** out) {
+  API_BEGIN
+  WhirParams wp;
+  wp.degree = p->degree;
+  wp.security_level = p->security_level;
+  wp.pow_bits = p->pow_bits;
+  wp.initial_folding_factor = p->initial_folding_factor;
+  wp.folding_factor = p->folding_factor;
+  wp.unique_decoding = p->unique_decoding != 0;
+  wp.starting_log_inv_rate = p->starting_log_inv_rate;
+  wp.batch_size = p->batch_size;
+  wp.hash_engine = p->hash_engine;
+  WHIR_REQUIRE(engine_supports_size(wp.hash_engine, 64), "hash engine cannot hash Merkle nodes");
+  auto* c = new whir_config{WhirCfg::make(size, wp)};
+  *out = c;
+  API_END
+}
+
+int whir_config_disable_pow(whir_config* c) {
+  API_BEGIN
+  c->cfg.disable_pow();
+  API_END
+}
+
+int whir_config_describe(const whir_config* c, char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  std::string s = c->cfg.describe();
+  *len = s.size();
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, s.size());
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  API_END
+}
+
+void whir_config_free(whir_config* c) { delete c; }
+
+int whir_transcript_new(const uint8_t protocol_id[64], const uint8_t session_id[32], const uint8_t* instance,
+                        size_t len, whir_transcript** out) {
+  API_BEGIN
+  auto* t = new whir_transcript;
+  auto d = std::make_unique<Duplex>(protocol_id, session_id, instance, len);
+  t->duplex = d.get();
+  t->t = std::move(d);
+  *out = t;
+  API_END
+}
+
+int whir_transcript_from_vtable(const whir_transcript_vtable* vt, whir_transcript** out) {
+  API_BEGIN
+  WHIR_REQUIRE(vt && vt->prover_field && vt->verifier_field && vt->verifier_bytes && vt->prover_hash &&
+                   vt->prover_u64 && vt->hint_hash && vt->hint_field_vec,
+               "transcript vtable: every callback must be set");
+  auto* t = new whir_transcript;
+  t->t = std::make_unique<VtableTranscript>(*vt);
+  *out = t;
+  API_END
+}
+
+int whir_transcript_proof(const whir_transcript* t, uint8_t* narg, size_t narg_cap, size_t* narg_len,
+                          uint8_t* hints, size_t hints_cap, size_t* hints_len) {
+  API_BEGIN
+  WHIR_REQUIRE(t->duplex, "transcript_proof: only the built-in transcript holds proof bytes");
+  const auto& a = t->duplex->narg();
+  const auto& h = t->duplex->hints();
+  *narg_len = a.size();
+  *hints_len = h.size();
+  if (narg) {
+    WHIR_REQUIRE(narg_cap >= a.size(), "transcript_proof: narg buffer too small");
+    memcpy(narg, a.data(), a.size());
+  }
+  if (hints) {
+    WHIR_REQUIRE(hints_cap >= h.size(), "transcript_proof: hints buffer too small");
+    memcpy(hints, h.data(), h.size());
+  }
+  API_END
+}
+
+void whir_transcript_free(whir_transcript* t) { delete t; }
+
+int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, whir_witness** out) {
+  API_BEGIN
+  hipStream_t s = thread_stream();
+  auto w = std::make_unique<whir_witness>();
+  switch (c->cfg.D) {
+    case 1: commit_impl<1>(c->cfg, *t->t, vector, w->w, s); break;
+    case 2: commit_impl<2>(c->cfg, *t->t, vector, w->w, s); break;
+    case 3: commit_impl<3>(c->cfg, *t->t, vector, w->w, s); break;
+  }
+  *out = w.release();
+  API_END
+}
+
+int whir_prove_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, const whir_witness* w,
+                   const uint64_t* points, size_t np, const uint64_t* evals, uint64_t* eval_point) {
+  API_BEGIN
+  hipStream_t s = thread_stream();
+  switch (c->cfg.D) {
+    case 1: prove_impl<1>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
+    case 2: prove_impl<2>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
+    case 3: prove_impl<3>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
+  }
+  API_END
+}
+
+void whir_witness_free(whir_witness* w) { delete w; }
+
+int whir_mle_evaluate_dev(int D, const uint64_t* vector, size_t nv, const uint64_t* point, uint64_t* out) {
+  API_BEGIN
+  WHIR_REQUIRE(D >= 1 && D <= 3, "degree must be 1, 2 or 3");
+  hipStream_t s = thread_stream();
+  size_t n = (size_t)1 << nv;
+  DevBuf eq(n * D * 8, s), lv(n * D * 8, s), o(D * 8, s);
+  fill_zero(eq.ptr, n * D * 8, s);
+  uint64_t one[3] = {1, 0, 0};
+  eval_eq_dev(D, eq.as<uint64_t>(), point, nv, one, s);
+  lift_dev(D, vector, n, lv.as<uint64_t>(), s);
+  dot_dev(D, lv.as<uint64_t>(), eq.as<uint64_t>(), n, o.as<uint64_t>(), s);
+  WHIR_HIP(hipMemcpyAsync(out, o.ptr, D * 8, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+}  // extern "C"

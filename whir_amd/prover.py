@@ -1,0 +1,144 @@
+"""WHIR prover binding: whir::Config (src/protocols/whir/config.rs:16-152), commit
+(src/protocols/whir/mod.rs:79-92) and prove (src/protocols/whir/prover.rs:50-316)
+running on the GPU through libwhir_amd.so."""
+from __future__ import annotations
+
+import ctypes
+import json
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (DeviceBuffer, HASH_BLAKE3, _call, _p64, _p8, ci, lib, sz, u64p, vp)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "degree", "security_level", "pow_bits", "initial_folding_factor", "folding_factor",
+        "unique_decoding", "starting_log_inv_rate", "batch_size", "hash_engine")]
+
+
+@dataclass
+class ProtocolParameters:
+    """ProtocolParameters (src/parameters.rs:8-24) plus the field degree
+    (1 = Goldilocks1, 2 = Goldilocks2, 3 = Goldilocks3)."""
+    degree: int = 2
+    security_level: int = 100
+    pow_bits: int = 20
+    initial_folding_factor: int = 4
+    folding_factor: int = 4
+    unique_decoding: bool = False
+    starting_log_inv_rate: int = 1
+    batch_size: int = 1
+    hash_engine: int = HASH_BLAKE3
+
+    def _c(self):
+        return _Params(self.degree, self.security_level, self.pow_bits, self.initial_folding_factor,
+                       self.folding_factor, int(self.unique_decoding), self.starting_log_inv_rate,
+                       self.batch_size, self.hash_engine)
+
+
+class Config:
+    """whir::Config::new(size, params)."""
+
+    def __init__(self, size: int, params: ProtocolParameters):
+        self.params = params
+        self.size = size
+        self.degree = params.degree
+        self.num_variables = size.bit_length() - 1
+        h = vp()
+        p = params._c()
+        _call("whir_config_new", ctypes.byref(p), size, ctypes.byref(h))
+        self.h = h.value
+
+    def disable_pow(self):
+        _call("whir_config_disable_pow", self.h)
+        return self
+
+    def describe(self) -> dict:
+        n = sz()
+        _call("whir_config_describe", self.h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _call("whir_config_describe", self.h, buf, n.value + 1, ctypes.byref(n))
+        return json.loads(buf.value.decode())
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().whir_config_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Transcript:
+    """The built-in Keccak duplex transcript (DomainSeparator protocol/session ids)."""
+
+    def __init__(self, protocol_id: bytes, session_id: bytes, instance: bytes = b""):
+        assert len(protocol_id) == 64 and len(session_id) == 32
+        self._keep = [np.frombuffer(bytes(x) + b"\0", dtype=np.uint8).copy()
+                      for x in (protocol_id, session_id, instance)]
+        h = vp()
+        _call("whir_transcript_new", _p8(self._keep[0]), _p8(self._keep[1]), _p8(self._keep[2]), len(instance),
+              ctypes.byref(h))
+        self.h = h.value
+
+    def proof(self):
+        """(narg_string, hints) of Proof (src/transcript/mod.rs:63-70)."""
+        na, nh = sz(), sz()
+        _call("whir_transcript_proof", self.h, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh))
+        a = np.zeros(max(na.value, 1), dtype=np.uint8)
+        b = np.zeros(max(nh.value, 1), dtype=np.uint8)
+        _call("whir_transcript_proof", self.h, _p8(a), a.size, ctypes.byref(na), _p8(b), b.size, ctypes.byref(nh))
+        return a[: na.value].tobytes(), b[: nh.value].tobytes()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().whir_transcript_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Witness:
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().whir_witness_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def commit(cfg: Config, t: Transcript, d_vector: DeviceBuffer) -> Witness:
+    h = vp()
+    _call("whir_commit_dev", cfg.h, t.h, d_vector.ptr, ctypes.byref(h))
+    return Witness(h.value)
+
+
+def prove(cfg: Config, t: Transcript, d_vector: DeviceBuffer, w: Witness, points, evaluations) -> np.ndarray:
+    """points: (num_points, num_variables * degree) words; evaluations: (num_points, degree)."""
+    D, nv = cfg.degree, cfg.num_variables
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.uint64)).reshape(-1)
+    ev = np.ascontiguousarray(np.asarray(evaluations, dtype=np.uint64)).reshape(-1)
+    npts = pts.size // max(nv * D, 1) if nv else len(evaluations) if len(evaluations) else 0
+    if pts.size == 0:
+        pts = np.zeros(1, dtype=np.uint64)
+    if ev.size == 0:
+        ev = np.zeros(1, dtype=np.uint64)
+    out = np.zeros(max(nv * D, 1), dtype=np.uint64)
+    _call("whir_prove_dev", cfg.h, t.h, d_vector.ptr, w.h, _p64(pts), npts, _p64(ev), _p64(out))
+    return out[: nv * D]
+
+
+def mle_evaluate(degree: int, d_vector: DeviceBuffer, num_variables: int, point) -> np.ndarray:
+    pt = np.ascontiguousarray(np.asarray(point, dtype=np.uint64)).reshape(-1)
+    if pt.size == 0:
+        pt = np.zeros(1, dtype=np.uint64)
+    out = np.zeros(degree, dtype=np.uint64)
+    _call("whir_mle_evaluate_dev", degree, d_vector.ptr, num_variables, _p64(pt), _p64(out))
+    return out
