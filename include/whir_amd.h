@@ -59,6 +59,15 @@ int whir_memcpy_d2d(void *dst, const void *src, size_t bytes);
 /* Wait for all work enqueued by the calling thread. */
 int whir_synchronize(void);
 
+/* Kernel timing with HIP events recorded on the launching stream (no reference
+ * counterpart; the reference's `tracing` spans play this role).  When enabled,
+ * every kernel family records launches, elapsed ms and algorithmic HBM bytes;
+ * whir_profile_report writes them as JSON ({"name": {"launches", "total_ms",
+ * "bytes"}}); pass buf = NULL to query the length. */
+int whir_profile_enable(int on);
+int whir_profile_reset(void);
+int whir_profile_report(char *buf, size_t cap, size_t *len);
+
 /* -------------------------------------------- ReedSolomon<F> (interleaved RS)
  * Replaces the trait ReedSolomon<F> (src/algebra/ntt/mod.rs:65-100) and the free
  * functions next_order / generator / evaluation_points / interleaved_rs_encode

@@ -174,6 +174,20 @@ def test_geometric_accumulate_matches_oracle(D, n, m):
     assert np.array_equal(W.geometric_accumulate(D, acc, s, x), O.geometric_accumulate(D, acc, s, x))
 
 
+@pytest.mark.parametrize("D", [2, 3])
+def test_geometric_accumulate_mixed_base_and_extension_points(D):
+    # STIR constraints mix extension (OOD) points with base-field (in-domain) points.
+    n, m = 70000, 9
+    acc = rnd_np(11, n * D)
+    s = rnd_np(12, m * D)
+    x = rnd_np(13, m * D).reshape(m, D)
+    x[1:, 1:] = 0  # points 1..8 lie in the base field
+    x[5] = 0       # and one is zero
+    x[6, 0] = 1    # and one is one
+    x = x.reshape(-1)
+    assert np.array_equal(W.geometric_accumulate(D, acc, s, x), O.geometric_accumulate(D, acc, s, x))
+
+
 @pytest.mark.parametrize("D", [1, 2, 3])
 @pytest.mark.parametrize("n", [0, 1, 2, 100, 4096, 70001])
 def test_univariate_evaluate_matches_oracle(D, n):

@@ -61,6 +61,9 @@ SIGNATURES = {
     "whir_memcpy_d2h": [vp, vp, sz],
     "whir_memcpy_d2d": [vp, vp, sz],
     "whir_synchronize": [],
+    "whir_profile_enable": [ci],
+    "whir_profile_reset": [],
+    "whir_profile_report": [ctypes.c_char_p, sz, szp],
     "whir_rs_next_order": [u64, u64p],
     "whir_rs_generator": [u64, u64p],
     "whir_rs_evaluation_points": [u64, u64, u64p, sz, u64p],
@@ -431,6 +434,24 @@ class DeviceBuffer:
 
 def synchronize():
     _call("whir_synchronize")
+
+
+def profile_enable(on: bool = True):
+    _call("whir_profile_enable", 1 if on else 0)
+
+
+def profile_reset():
+    _call("whir_profile_reset")
+
+
+def profile_report() -> dict:
+    """{kernel family: {"launches", "total_ms", "bytes"}} from HIP events on the launching stream."""
+    import json
+    n = sz()
+    _call("whir_profile_report", None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _call("whir_profile_report", buf, n.value + 1, ctypes.byref(n))
+    return json.loads(buf.value.decode())
 
 
 def rs_encode_dev(degree: int, d_msgs: DeviceBuffer, num_messages: int, message_length: int,

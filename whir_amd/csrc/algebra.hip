@@ -128,6 +128,7 @@ static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* ou
   size_t half = next_pow2(n) >> 1, nhi = n - half;
   unsigned g = grid_for(half);
   DevBuf parts((size_t)g * 2 * D * 8, s);
+  ProfScope prof("sumcheck_poly", s, 16.0 * D * (double)n);
   k_sc_poly<D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
   WHIR_HIP(hipGetLastError());
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
@@ -162,6 +163,7 @@ static size_t fold_t(uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) {
   size_t half = next_pow2(n) >> 1, nhi = n - half;
   Ext<D> ww = host_ext<D>(w);
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
+  ProfScope prof("fold", s, 8.0 * D * ((double)n + (double)half));
   k_fold<D><<<grid_for(half), RT, 0, s>>>(v, half, nhi, ww, om);
   WHIR_HIP(hipGetLastError());
   return half;
@@ -216,6 +218,7 @@ static size_t fold_poly_t(uint64_t* a, uint64_t* b, size_t n, const uint64_t* w,
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
   unsigned g = grid_for(h >> 1);
   DevBuf parts((size_t)g * 2 * D * 8, s);
+  ProfScope prof("fold_sumcheck_poly", s, 8.0 * D * (2.0 * n + 2.0 * h));
   k_fold_poly<D><<<g, RT, 0, s>>>(a, b, h, nhi, ww, om, parts.as<uint64_t>());
   WHIR_HIP(hipGetLastError());
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
@@ -271,13 +274,12 @@ static void eval_eq_t(uint64_t* acc, const uint64_t* z, size_t k, const uint64_t
   auto hi = eq_table<D>(z, k1, host_ext<D>(scalar));
   auto lo = eq_table<D>(z + k1 * D, k2, gl::ext_base<D>(1));
   DevBuf dh(hi.size() * 8, s), dl(lo.size() * 8, s);
-  WHIR_HIP(hipMemcpyAsync(dh.ptr, hi.data(), hi.size() * 8, hipMemcpyHostToDevice, s));
-  WHIR_HIP(hipMemcpyAsync(dl.ptr, lo.data(), lo.size() * 8, hipMemcpyHostToDevice, s));
+  upload_async(dh.ptr, hi.data(), hi.size() * 8, s);
+  upload_async(dl.ptr, lo.data(), lo.size() * 8, s);
   size_t n = (size_t)1 << k;
+  ProfScope prof("eval_eq", s, 16.0 * D * (double)n);
   k_eval_eq<D><<<grid_for(n), RT, 0, s>>>(acc, n, dh.as<uint64_t>(), dl.as<uint64_t>(), (int)k2);
   WHIR_HIP(hipGetLastError());
-  // The staging copies read host memory asynchronously; wait before it goes away.
-  WHIR_HIP(hipStreamSynchronize(s));
 }
 
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t s) {
@@ -295,21 +297,35 @@ void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const ui
 // xb[j*nb + b] = s_j x_j^(b RT K), xt[j*RT + t] = x_j^t, xs[j] = x_j^RT (host tables).
 static constexpr int GA_K = 16;
 
+// Points are split into extension points (first m_e of xb; tables xt_e, xs_e in
+// the extension) and base-field points (next m_b; tables xt_b, xs_b in the base
+// field, stepped with D base multiplications instead of a full extension product).
+// The in-domain STIR points are all base-field elements.
 template <int D>
-__global__ __launch_bounds__(RT) void k_geom(uint64_t* acc, size_t n, const uint64_t* xb, const uint64_t* xt,
-                                             const uint64_t* xs, size_t m, size_t nb) {
+__global__ __launch_bounds__(RT) void k_geom(uint64_t* acc, size_t n, const uint64_t* xb, const uint64_t* xt_e,
+                                             const uint64_t* xs_e, const uint64_t* xt_b, const uint64_t* xs_b,
+                                             size_t m_e, size_t m_b, size_t nb) {
   const size_t b = blockIdx.x;
   const size_t i0 = b * (size_t)RT * GA_K + threadIdx.x;
   Ext<D> sum[GA_K];
 #pragma unroll
   for (int k = 0; k < GA_K; k++) sum[k] = gl::ext_zero<D>();
-  for (size_t j = 0; j < m; j++) {
-    Ext<D> p = gl::emul<D>(ld<D>(xb, j * nb + b), ld<D>(xt, j * RT + threadIdx.x));
-    Ext<D> step = ld<D>(xs, j);
+  for (size_t j = 0; j < m_e; j++) {
+    Ext<D> p = gl::emul<D>(ld<D>(xb, j * nb + b), ld<D>(xt_e, j * RT + threadIdx.x));
+    Ext<D> step = ld<D>(xs_e, j);
 #pragma unroll
     for (int k = 0; k < GA_K; k++) {
       sum[k] = gl::eadd<D>(sum[k], p);
       p = gl::emul<D>(p, step);
+    }
+  }
+  for (size_t j = 0; j < m_b; j++) {
+    Ext<D> p = gl::escale<D>(ld<D>(xb, (m_e + j) * nb + b), xt_b[j * RT + threadIdx.x]);
+    const uint64_t step = xs_b[j];
+#pragma unroll
+    for (int k = 0; k < GA_K; k++) {
+      sum[k] = gl::eadd<D>(sum[k], p);
+      p = gl::escale<D>(p, step);
     }
   }
 #pragma unroll
@@ -325,30 +341,50 @@ static void geom_t(uint64_t* acc, size_t n, const uint64_t* scalars, const uint6
   if (!n || !m) return;
   size_t span = (size_t)RT * GA_K;
   size_t nb = (n + span - 1) / span;
-  std::vector<uint64_t> xb(m * nb * D), xt(m * RT * D), xs(m * D);
+  // Extension points first, then points that lie in the base field.
+  std::vector<size_t> order_e, order_b;
   for (size_t j = 0; j < m; j++) {
+    bool base = true;
+    for (int d = 1; d < D; d++) base = base && points[j * D + d] == 0;
+    (base && D > 1 ? order_b : order_e).push_back(j);
+  }
+  const size_t m_e = order_e.size(), m_b = order_b.size();
+  std::vector<uint64_t> xb(m * nb * D), xt_e(std::max<size_t>(m_e, 1) * RT * D), xs_e(std::max<size_t>(m_e, 1) * D);
+  std::vector<uint64_t> xt_b(std::max<size_t>(m_b, 1) * RT), xs_b(std::max<size_t>(m_b, 1));
+  size_t slot = 0;
+  for (size_t q = 0; q < m_e + m_b; q++, slot++) {
+    const bool is_base = q >= m_e;
+    const size_t j = is_base ? order_b[q - m_e] : order_e[q];
     Ext<D> x = host_ext<D>(points + j * D);
     Ext<D> sj = host_ext<D>(scalars + j * D);
     Ext<D> xspan = gl::epow<D>(x, span);
     Ext<D> cur = sj;
     for (size_t b = 0; b < nb; b++) {
-      for (int d = 0; d < D; d++) xb[(j * nb + b) * D + d] = cur.c[d];
+      for (int d = 0; d < D; d++) xb[(slot * nb + b) * D + d] = cur.c[d];
       cur = gl::emul<D>(cur, xspan);
     }
     Ext<D> pw = gl::ext_base<D>(1);
     for (int t = 0; t < RT; t++) {
-      for (int d = 0; d < D; d++) xt[(j * RT + t) * D + d] = pw.c[d];
+      if (is_base) xt_b[(q - m_e) * RT + t] = pw.c[0];
+      else
+        for (int d = 0; d < D; d++) xt_e[(q * RT + t) * D + d] = pw.c[d];
       pw = gl::emul<D>(pw, x);
     }
-    for (int d = 0; d < D; d++) xs[j * D + d] = pw.c[d];  // x^RT
+    if (is_base) xs_b[q - m_e] = pw.c[0];  // x^RT
+    else
+      for (int d = 0; d < D; d++) xs_e[q * D + d] = pw.c[d];
   }
-  DevBuf db(xb.size() * 8, s), dt(xt.size() * 8, s), ds(xs.size() * 8, s);
-  WHIR_HIP(hipMemcpyAsync(db.ptr, xb.data(), xb.size() * 8, hipMemcpyHostToDevice, s));
-  WHIR_HIP(hipMemcpyAsync(dt.ptr, xt.data(), xt.size() * 8, hipMemcpyHostToDevice, s));
-  WHIR_HIP(hipMemcpyAsync(ds.ptr, xs.data(), xs.size() * 8, hipMemcpyHostToDevice, s));
-  k_geom<D><<<(unsigned)nb, RT, 0, s>>>(acc, n, db.as<uint64_t>(), dt.as<uint64_t>(), ds.as<uint64_t>(), m, nb);
+  DevBuf db(xb.size() * 8, s), dte(xt_e.size() * 8, s), dse(xs_e.size() * 8, s), dtb(xt_b.size() * 8, s),
+      dsb(xs_b.size() * 8, s);
+  upload_async(db.ptr, xb.data(), xb.size() * 8, s);
+  upload_async(dte.ptr, xt_e.data(), xt_e.size() * 8, s);
+  upload_async(dse.ptr, xs_e.data(), xs_e.size() * 8, s);
+  upload_async(dtb.ptr, xt_b.data(), xt_b.size() * 8, s);
+  upload_async(dsb.ptr, xs_b.data(), xs_b.size() * 8, s);
+  ProfScope prof("geometric_accumulate", s, 16.0 * D * (double)n);
+  k_geom<D><<<(unsigned)nb, RT, 0, s>>>(acc, n, db.as<uint64_t>(), dte.as<uint64_t>(), dse.as<uint64_t>(),
+                                        dtb.as<uint64_t>(), dsb.as<uint64_t>(), m_e, m_b, nb);
   WHIR_HIP(hipGetLastError());
-  WHIR_HIP(hipStreamSynchronize(s));
 }
 
 void geometric_accumulate_dev(int D, uint64_t* acc, size_t n, const uint64_t* scalars, const uint64_t* points,
@@ -390,8 +426,7 @@ template <int Ds, int D>
 static void uni_t(const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out, hipStream_t s) {
   if (!n) {
     std::vector<uint64_t> z(D, 0);
-    WHIR_HIP(hipMemcpyAsync(out, z.data(), D * 8, hipMemcpyHostToDevice, s));
-    WHIR_HIP(hipStreamSynchronize(s));
+    upload_async(out, z.data(), D * 8, s);
     return;
   }
   size_t span = (size_t)RT * GA_K;
@@ -410,13 +445,15 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out,
     pw = gl::emul<D>(pw, xx);
   }
   DevBuf db(xb.size() * 8, s), dt(xt.size() * 8, s), parts(nb * D * 8, s);
-  WHIR_HIP(hipMemcpyAsync(db.ptr, xb.data(), xb.size() * 8, hipMemcpyHostToDevice, s));
-  WHIR_HIP(hipMemcpyAsync(dt.ptr, xt.data(), xt.size() * 8, hipMemcpyHostToDevice, s));
-  k_uni<Ds, D><<<(unsigned)nb, RT, 0, s>>>(c, n, xspan, db.as<uint64_t>(), dt.as<uint64_t>(), pw,
-                                           parts.as<uint64_t>());
-  WHIR_HIP(hipGetLastError());
+  upload_async(db.ptr, xb.data(), xb.size() * 8, s);
+  upload_async(dt.ptr, xt.data(), xt.size() * 8, s);
+  {
+    ProfScope prof("univariate_eval", s, 8.0 * Ds * (double)n);
+    k_uni<Ds, D><<<(unsigned)nb, RT, 0, s>>>(c, n, xspan, db.as<uint64_t>(), dt.as<uint64_t>(), pw,
+                                             parts.as<uint64_t>());
+    WHIR_HIP(hipGetLastError());
+  }
   finish_reduce<D, 1>(parts.as<uint64_t>(), (unsigned)nb, out, s);
-  WHIR_HIP(hipStreamSynchronize(s));
 }
 
 void univariate_eval_dev(int Ds, int D, const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out,
@@ -445,6 +482,7 @@ __global__ void k_lift(const uint64_t* src, size_t n, uint64_t* dst) {
 
 void lift_dev(int D, const uint64_t* src, size_t n, uint64_t* dst, hipStream_t s) {
   if (!n) return;
+  ProfScope prof("lift", s, 8.0 * (1.0 + D) * (double)n);
   switch (D) {
     case 1: k_lift<1><<<grid_for(n), RT, 0, s>>>(src, n, dst); break;
     case 2: k_lift<2><<<grid_for(n), RT, 0, s>>>(src, n, dst); break;

@@ -147,6 +147,30 @@ int whir_synchronize(void) {
   API_END
 }
 
+int whir_profile_enable(int on) {
+  API_BEGIN
+  profile_set(on != 0);
+  API_END
+}
+
+int whir_profile_reset(void) {
+  API_BEGIN
+  profile_reset();
+  API_END
+}
+
+int whir_profile_report(char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  std::string s = profile_json();
+  *len = s.size();
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, s.size());
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  API_END
+}
+
 // ------------------------------------------------------------------ ReedSolomon
 int whir_rs_next_order(uint64_t size, uint64_t* out) {
   API_BEGIN

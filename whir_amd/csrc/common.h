@@ -35,14 +35,29 @@ const char* last_error();
 // on first use; one per thread so concurrent callers overlap on the device).
 hipStream_t thread_stream();
 
-// Stream-ordered device scratch that frees itself on scope exit.
+// Enqueue a host->device copy on stream s; `src` may be reused as soon as this
+// returns (small copies go through a pinned per-thread staging area).
+void upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Copy device->host through a pinned per-thread buffer and wait for it.
+void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Device buffers recycled per stream: a buffer released on stream s goes to s's
+// free list and is handed out again only to allocations on s, so stream order
+// alone keeps its last readers ahead of its next writers.  Sizes repeat from
+// one proof to the next, so after the first proof nothing is allocated.
+void* pool_alloc(size_t bytes, hipStream_t s);
+void pool_free(void* p, size_t bytes, hipStream_t s);
+// Drop every cached buffer of stream s (e.g. before the stream is destroyed).
+void pool_trim(hipStream_t s);
+
+// Stream-ordered device scratch that returns to the pool on scope exit.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
   hipStream_t stream = nullptr;
   DevBuf() = default;
   DevBuf(size_t n, hipStream_t s) : bytes(n), stream(s) {
-    if (n) WHIR_HIP(hipMallocAsync(&ptr, n, s));
+    if (n) ptr = pool_alloc(n, s);
   }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -54,7 +69,7 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   void release() {
-    if (ptr) (void)hipFreeAsync(ptr, stream);
+    if (ptr) pool_free(ptr, bytes, stream);
     ptr = nullptr;
   }
   template <class T> T* as() const { return static_cast<T*>(ptr); }
@@ -67,5 +82,21 @@ inline unsigned blocks_for(size_t n, unsigned threads) {
 
 // Zero a device range with a kernel (no memset nodes: the launch may be captured).
 void fill_zero(void* dst, size_t bytes, hipStream_t s);
+
+// Kernel timing with HIP events on the launching stream, off unless enabled
+// (whir_profile_enable).  A scope brackets one launch of a named kernel family
+// and credits it with its algorithmic HBM bytes.
+bool profile_on();
+void profile_set(bool on);
+void profile_reset();
+std::string profile_json();  // {"name": {"launches", "total_ms", "bytes"}}
+struct ProfScope {
+  ProfScope(const char* name, hipStream_t s, double bytes);
+  ~ProfScope();
+  const char* name;
+  hipStream_t s;
+  double bytes;
+  hipEvent_t start = nullptr;
+};
 
 }  // namespace whir

@@ -107,6 +107,7 @@ template <int D>
 static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st) {
   size_t W = M * D;
   int RB = (int)std::min<size_t>(LEAF_THREADS, 8192 / (W + 1));
+  ProfScope prof("leaf_hash", st, 8.0 * (double)W * N + 32.0 * N);
   if (RB >= 16) {
     size_t smem = (size_t)RB * (W + 1) * 8;
     k_leaves_cols<D><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(engine, cw, N, M, leaves, RB);
@@ -217,6 +218,7 @@ void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t
     while (lv < maxlev && ((size_t)1 << (lv + 1)) <= w) lv++;
     size_t blocks = w >> lv;
     int threads = std::max(64, std::min(MERKLE_THREADS, 1 << (lv - 1)));
+    ProfScope prof("merkle", st, 32.0 * (double)(2 * w - (w >> lv)));
     k_merkle_levels<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, lv);
     WHIR_HIP(hipGetLastError());
     w >>= lv;
@@ -301,12 +303,12 @@ bool pow_grind_dev(int engine, const uint8_t challenge[32], uint64_t threshold, 
   WHIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   uint64_t num_chunks = count / POW_CHUNK;
   unsigned blocks = (unsigned)std::min<uint64_t>((uint64_t)cus * 8, num_chunks);
+  ProfScope prof("pow", st, 0.0);
   k_pow<<<blocks, POW_THREADS, 0, st>>>(engine, make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]),
                                         threshold, first, num_chunks, state.as<unsigned long long>());
   WHIR_HIP(hipGetLastError());
   unsigned long long host = 0;
-  WHIR_HIP(hipMemcpyAsync(&host, state.ptr, 8, hipMemcpyDeviceToHost, st));
-  WHIR_HIP(hipStreamSynchronize(st));
+  download_sync(&host, state.ptr, 8, st);
   if (host == ~0ull) return false;
   *nonce = host;
   return true;

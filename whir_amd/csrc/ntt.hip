@@ -191,7 +191,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, TwTable tw
 }
 
 // --------------------------------------------------------------------------
-// Pass 1: columns.  Block = (b consecutive i1, message j); loops over cosets.
+// Pass 1: columns.  Block = (b consecutive i1, coset c, message j); the Q blocks
+// of one column tile read the same input, which L2 / MALL serve after the first.
 // Writes Y (same shape as the codeword) at [j][c*C + t2*C1 + i1].
 // --------------------------------------------------------------------------
 template <int D>
@@ -205,12 +206,11 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, TwTable twN
   uint64_t* tws = smem + (size_t)b * C2 * D;
   for (int k = threadIdx.x; k < (C2 >> 1); k += blockDim.x) tws[k] = twS[k];
   const int i1_0 = blockIdx.x * b;
-  const size_t j = blockIdx.y;
+  const size_t c = blockIdx.y, j = blockIdx.z;
   const uint64_t* msg = a.msgs + j * a.msg_stride * D;
   const uint64_t* mask = a.masks ? a.masks + j * a.mask_len * D : nullptr;
   const int tile = b * C2;
-  for (size_t c = 0; c < a.Q; c++) {
-    __syncthreads();
+  {
     for (int e = threadIdx.x; e < tile; e += blockDim.x) {
       int col = e % b, i2 = e / b;
       size_t i = (size_t)(i1_0 + col) + (size_t)C1 * i2;
@@ -317,6 +317,7 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     const uint64_t* twS = flat_twiddles(logC);
     size_t smem = ((size_t)G * C * D + std::max(1, C / 2)) * 8;
     unsigned blocks = (unsigned)((nseq + G - 1) / G);
+    ProfScope prof("ntt_single", st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
     k_ntt_single<D><<<blocks, NTT_THREADS, smem, st>>>(a, twN, twS, G);
     WHIR_HIP(hipGetLastError());
     return;
@@ -329,7 +330,8 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
   {
     int b = std::min(C1, std::max(1, floor_pow2(TILE_WORDS / (C2 * D))));
     size_t smem = ((size_t)b * C2 * D + C2 / 2) * 8;
-    dim3 grid(C1 / b, a.M);
+    dim3 grid(C1 / b, a.Q, a.M);
+    ProfScope prof("ntt_pass1", st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
     k_ntt_pass1<D><<<grid, NTT_THREADS, smem, st>>>(a, twN, twC, flat_twiddles(logC2), scratch, logC1, b);
     WHIR_HIP(hipGetLastError());
   }
@@ -338,6 +340,7 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     int b = std::min(C2, std::max(1, floor_pow2(TILE_WORDS / (C1 * D))));
     size_t smem = ((size_t)b * C1 * D + C1 / 2) * 8;
     dim3 grid(C2 / b, a.Q, a.M);
+    ProfScope prof("ntt_pass2", st, 16.0 * D * (double)a.M * a.N);
     k_ntt_pass2<D><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, b);
     WHIR_HIP(hipGetLastError());
   }

@@ -152,8 +152,7 @@ uint64_t coset_size(uint64_t masked_len, uint64_t N) {
 
 template <int D>
 void d2h(E<D>& out, const uint64_t* dev, hipStream_t s) {
-  WHIR_HIP(hipMemcpyAsync(out.c, dev, D * 8, hipMemcpyDeviceToHost, s));
-  WHIR_HIP(hipStreamSynchronize(s));
+  download_sync(out.c, dev, D * 8, s);
 }
 
 // irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (one vector,
@@ -178,8 +177,7 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec,
   hash_rows_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s);
   merkle_build_dev(engine, w.nodes.as<uint8_t>(), N, s);
   uint8_t root[32];
-  WHIR_HIP(hipMemcpyAsync(root, w.nodes.as<uint8_t>() + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, s));
-  WHIR_HIP(hipStreamSynchronize(s));
+  download_sync(root, w.nodes.as<uint8_t>() + 32 * (nn - 1), 32, s);
   t.prover_hash(root);
   w.ood_points.clear();
   w.ood_values.clear();
@@ -209,10 +207,9 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
   if (k) {
     std::vector<uint64_t> idx64(idx.begin(), idx.end());
     DevBuf didx(k * 8, s), drows(k * M * w.Ds * 8, s);
-    WHIR_HIP(hipMemcpyAsync(didx.ptr, idx64.data(), k * 8, hipMemcpyHostToDevice, s));
+    upload_async(didx.ptr, idx64.data(), k * 8, s);
     gather_rows_dev(w.Ds, w.cw.as<uint64_t>(), N, M, didx.as<uint64_t>(), k, drows.as<uint64_t>(), s);
-    WHIR_HIP(hipMemcpyAsync(rows.data(), drows.ptr, k * M * w.Ds * 8, hipMemcpyDeviceToHost, s));
-    WHIR_HIP(hipStreamSynchronize(s));
+    download_sync(rows.data(), drows.ptr, k * M * w.Ds * 8, s);
   }
   t.hint_field_vec(w.Ds, rows.data(), k * M);
   // merkle_tree::Config::open: sibling node indices in hint order
@@ -240,10 +237,9 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
   if (!sib.empty()) {
     std::vector<uint8_t> hashes(32 * sib.size());
     DevBuf dsib(sib.size() * 8, s), dh(32 * sib.size(), s);
-    WHIR_HIP(hipMemcpyAsync(dsib.ptr, sib.data(), sib.size() * 8, hipMemcpyHostToDevice, s));
+    upload_async(dsib.ptr, sib.data(), sib.size() * 8, s);
     gather_hashes_dev(w.nodes.as<uint8_t>(), dsib.as<uint64_t>(), sib.size(), dh.as<uint8_t>(), s);
-    WHIR_HIP(hipMemcpyAsync(hashes.data(), dh.ptr, hashes.size(), hipMemcpyDeviceToHost, s));
-    WHIR_HIP(hipStreamSynchronize(s));
+    download_sync(hashes.data(), dh.ptr, hashes.size(), s);
     for (size_t i = 0; i < sib.size(); i++) t.hint_hash(&hashes[32 * i]);
   }
 }
@@ -261,8 +257,7 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     if (have_prev) n = fold_and_poly_dev(D, a, b, n, prev.c, dout.as<uint64_t>(), s);
     else sumcheck_poly_dev(D, a, b, n, dout.as<uint64_t>(), s);
     uint64_t h[2 * D];
-    WHIR_HIP(hipMemcpyAsync(h, dout.ptr, 2 * D * 8, hipMemcpyDeviceToHost, s));
-    WHIR_HIP(hipStreamSynchronize(s));
+    download_sync(h, dout.ptr, 2 * D * 8, s);
     E<D> c0 = ld<D>(h), c2 = ld<D>(h + D);
     E<D> c1 = gl::esub<D>(gl::esub<D>(gl::esub<D>(sum, c0), c0), c2);
     t.prover_field(D, c0.c, 1);
@@ -384,8 +379,7 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
   // Send the final vector directly.
   WHIR_REQUIRE(n == cfg.final_sumcheck.initial_size, "final vector length mismatch");
   std::vector<uint64_t> fin(n * D);
-  WHIR_HIP(hipMemcpyAsync(fin.data(), vec.ptr, n * D * 8, hipMemcpyDeviceToHost, s));
-  WHIR_HIP(hipStreamSynchronize(s));
+  download_sync(fin.data(), vec.ptr, n * D * 8, s);
   for (size_t i = 0; i < n; i++) t.prover_field(D, &fin[i * D], 1);
   pow_prove(t, eng, cfg.final_pow, s);
   {

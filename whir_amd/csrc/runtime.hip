@@ -1,7 +1,13 @@
 // Runtime plumbing: last-error storage, per-thread streams, zero fill.
 #include "common.h"
 
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace whir {
@@ -16,21 +22,164 @@ struct ThreadStream {
   hipStream_t s = nullptr;
   int device = -1;
   ~ThreadStream() {
-    if (s) (void)hipStreamDestroy(s);
+    if (s) {
+      pool_trim(s);
+      (void)hipStreamDestroy(s);
+    }
   }
 };
 thread_local ThreadStream t_stream;
 }  // namespace
+
+namespace {
+std::mutex g_pool_mu;
+std::vector<int> g_pool_ready;
+
+// Keep freed stream-ordered memory cached in the device pool: with the default
+// release threshold (0) every synchronization hands it back and the next proof
+// maps its half-GiB codewords again.
+void configure_pool(int dev) {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  for (int d : g_pool_ready)
+    if (d == dev) return;
+  hipMemPool_t pool;
+  WHIR_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t thr = UINT64_MAX;
+  WHIR_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  g_pool_ready.push_back(dev);
+}
+
+// Per-thread pinned staging area for small host->device uploads, so they can be
+// enqueued without waiting for the stream.  Regions are reused only after the
+// whole area has wrapped and the copies enqueued before the wrap have finished.
+struct Staging {
+  uint8_t* host = nullptr;
+  size_t cap = 0, off = 0;
+  hipEvent_t wrap = nullptr;
+  int device = -1;
+  ~Staging() {
+    if (host) (void)hipHostFree(host);
+    if (wrap) (void)hipEventDestroy(wrap);
+  }
+};
+thread_local Staging t_staging;
+}  // namespace
+
+namespace {
+std::mutex g_cache_mu;
+// (stream, size rounded to 256 B) -> free buffers
+std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> g_cache;
+size_t round_bytes(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+void* pool_alloc(size_t bytes, hipStream_t s) {
+  size_t rb = round_bytes(bytes);
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto it = g_cache.find({s, rb});
+    if (it != g_cache.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+  }
+  void* p = nullptr;
+  WHIR_HIP(hipMallocAsync(&p, rb, s));
+  return p;
+}
+
+void pool_free(void* p, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  g_cache[{s, round_bytes(bytes)}].push_back(p);
+}
+
+void pool_trim(hipStream_t s) {
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+      if (it->first.first == s) {
+        drop.insert(drop.end(), it->second.begin(), it->second.end());
+        it = g_cache.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (void* p : drop) (void)hipFreeAsync(p, s);
+  if (!drop.empty()) (void)hipStreamSynchronize(s);
+}
 
 hipStream_t thread_stream() {
   int dev = 0;
   WHIR_HIP(hipGetDevice(&dev));
   if (!t_stream.s || t_stream.device != dev) {
     if (t_stream.s) (void)hipStreamDestroy(t_stream.s);
+    configure_pool(dev);
     WHIR_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
     t_stream.device = dev;
   }
   return t_stream.s;
+}
+
+void upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  Staging& st = t_staging;
+  int dev = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  if (!st.host || st.device != dev) {
+    if (st.host) (void)hipHostFree(st.host);
+    if (st.wrap) (void)hipEventDestroy(st.wrap);
+    st.cap = (size_t)8 << 20;
+    WHIR_HIP(hipHostMalloc((void**)&st.host, st.cap, hipHostMallocDefault));
+    WHIR_HIP(hipEventCreateWithFlags(&st.wrap, hipEventDisableTiming));
+    st.off = 0;
+    st.device = dev;
+  }
+  if (bytes > st.cap / 4) {  // large: plain copy, then wait so `src` may be released
+    WHIR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    WHIR_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  if (st.off + bytes > st.cap) {
+    WHIR_HIP(hipEventRecord(st.wrap, s));
+    WHIR_HIP(hipEventSynchronize(st.wrap));
+    st.off = 0;
+  }
+  memcpy(st.host + st.off, src, bytes);
+  WHIR_HIP(hipMemcpyAsync(dst, st.host + st.off, bytes, hipMemcpyHostToDevice, s));
+  st.off += (bytes + 255) & ~(size_t)255;
+}
+
+namespace {
+struct Readback {
+  uint8_t* host = nullptr;
+  size_t cap = 0;
+  int device = -1;
+  ~Readback() {
+    if (host) (void)hipHostFree(host);
+  }
+};
+thread_local Readback t_readback;
+}  // namespace
+
+void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) {
+    WHIR_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  Readback& rb = t_readback;
+  int dev = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  if (!rb.host || rb.device != dev || rb.cap < bytes) {
+    if (rb.host) (void)hipHostFree(rb.host);
+    rb.cap = std::max<size_t>(bytes, (size_t)1 << 20);
+    WHIR_HIP(hipHostMalloc((void**)&rb.host, rb.cap, hipHostMallocDefault));
+    rb.device = dev;
+  }
+  WHIR_HIP(hipMemcpyAsync(rb.host, src, bytes, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+  memcpy(dst, rb.host, bytes);
 }
 
 __global__ void k_fill_zero(uint4* __restrict__ p, size_t n16, uint8_t* __restrict__ tail, size_t ntail) {
@@ -38,6 +187,94 @@ __global__ void k_fill_zero(uint4* __restrict__ p, size_t n16, uint8_t* __restri
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t k = i; k < n16; k += stride) p[k] = make_uint4(0, 0, 0, 0);
   if (i < ntail) tail[i] = 0;
+}
+
+// ------------------------------------------------------------------ profiler
+namespace {
+struct ProfRecord {
+  hipEvent_t start, stop;
+  double bytes;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::map<std::string, std::vector<ProfRecord>> g_prof;
+std::vector<hipEvent_t> g_event_pool;  // recycled events (creating one per launch costs more than the launch)
+
+hipEvent_t take_event() {
+  {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!g_event_pool.empty()) {
+      hipEvent_t e = g_event_pool.back();
+      g_event_pool.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+bool profile_on() { return g_prof_on; }
+
+ProfScope::ProfScope(const char* n, hipStream_t st, double b) : name(n), s(st), bytes(b) {
+  if (!g_prof_on) return;
+  start = take_event();
+  if (start) (void)hipEventRecord(start, s);
+}
+
+ProfScope::~ProfScope() {
+  if (!start) return;
+  hipEvent_t stop = take_event();
+  if (!stop) return;
+  (void)hipEventRecord(stop, s);
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof[name].push_back({start, stop, bytes});
+}
+
+void profile_set(bool on) {
+  g_prof_on = on;
+  if (on) {  // pre-create a pool so the timed region records into existing events
+    std::vector<hipEvent_t> fresh;
+    for (int i = 0; i < 2048; i++) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) == hipSuccess) fresh.push_back(e);
+    }
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_event_pool.insert(g_event_pool.end(), fresh.begin(), fresh.end());
+  }
+}
+
+void profile_reset() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  for (auto& kv : g_prof)
+    for (auto& r : kv.second) {
+      (void)hipEventSynchronize(r.stop);
+      g_event_pool.push_back(r.start);
+      g_event_pool.push_back(r.stop);
+    }
+  g_prof.clear();
+}
+
+std::string profile_json() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  std::string out = "{";
+  bool first = true;
+  for (auto& kv : g_prof) {
+    double ms = 0, bytes = 0;
+    for (auto& r : kv.second) {
+      float e = 0;
+      (void)hipEventSynchronize(r.stop);
+      if (hipEventElapsedTime(&e, r.start, r.stop) == hipSuccess) ms += e;
+      bytes += r.bytes;
+    }
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s\"%s\": {\"launches\": %zu, \"total_ms\": %.6f, \"bytes\": %.0f}", first ? "" : ", ",
+             kv.first.c_str(), kv.second.size(), ms, bytes);
+    out += buf;
+    first = false;
+  }
+  return out + "}";
 }
 
 void fill_zero(void* dst, size_t bytes, hipStream_t s) {
