@@ -32,6 +32,20 @@ GL_FN uint64_t umulhi(uint64_t a, uint64_t b) {
 }
 
 GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Same steps with explicit carry chains, so every correction is a masked
+  // subtract or add instead of a 64-bit compare and two selects.
+  unsigned long c;
+  const uint64_t hh = hi >> 32;
+  const uint32_t hl = (uint32_t)hi;
+  uint64_t t0 = __builtin_subcl(lo, hh, 0, &c);                // lo - hh, c = borrow
+  t0 -= (uint64_t)(0u - (uint32_t)c);                          // borrow: subtract 2^32 - 1 (add p)
+  const uint64_t t1 = (uint64_t)hl * 0xFFFFFFFFu;              // hl * (2^32 - 1)
+  uint64_t t2 = __builtin_addcl(t0, t1, 0, &c);
+  t2 += (uint64_t)(0u - (uint32_t)c);                          // carry: 2^64 = 2^32 - 1
+  uint64_t t3 = __builtin_addcl(t2, EPS, 0, &c);               // t2 + 2^32 - 1 wraps iff t2 >= p
+  return c ? t3 : t2;
+#else
   uint64_t hh = hi >> 32;
   uint64_t hl = hi & EPS;
   uint64_t t0 = lo - hh;
@@ -41,6 +55,7 @@ GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
   if (t2 < t1) t2 += EPS;               // carry: 2^64 = EPS
   if (t2 >= P) t2 -= P;
   return t2;
+#endif
 }
 
 GL_FN uint64_t add(uint64_t a, uint64_t b) {
@@ -53,7 +68,30 @@ GL_FN uint64_t sub(uint64_t a, uint64_t b) {
   return a < b ? d - EPS : d;           // d + p mod 2^64
 }
 GL_FN uint64_t neg(uint64_t a) { return a ? P - a : 0; }
-GL_FN uint64_t mul(uint64_t a, uint64_t b) { return reduce128(a * b, umulhi(a, b)); }
+
+// 64x64 -> 128-bit product from four 32x32 -> 64 partial products, each carrying
+// the previous one's spill as its addend (v_mad_u64_u32 on CDNA): no partial sum
+// overflows 64 bits, and the low product is not recomputed.
+GL_FN void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t mid = (uint64_t)a0 * b1 + (p00 >> 32);           // <= 2^64 - 2^32
+  const uint64_t mid2 = (uint64_t)a1 * b0 + (uint32_t)mid;         // <  2^64
+  hi = (uint64_t)a1 * b1 + (mid >> 32) + (mid2 >> 32);              // <= 2^64 - 1
+  lo = (mid2 << 32) | (uint32_t)p00;
+#else
+  unsigned __int128 p = (unsigned __int128)a * b;
+  lo = (uint64_t)p;
+  hi = (uint64_t)(p >> 64);
+#endif
+}
+
+GL_FN uint64_t mul(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  return reduce128(lo, hi);
+}
 GL_FN uint64_t sqr(uint64_t a) { return mul(a, a); }
 
 // x * 2^k for 0 <= k < 192 (2 has order 192: 2^96 = -1).

@@ -10,13 +10,18 @@
 // column-major (message-major): cw[j*N + r].  Leaf hashing reads it in tiles of
 // whole rows, so the row-major matrix only has to be formed for host callers.
 //
-// DFT_C of length C = 2^n is a four-step transform over C = C1 * C2:
-//   pass 1, per column i1: Y[t2][i1] = w_C^{i1 t2} * sum_i2 x[i1 + C1 i2] w_C2^{i2 t2}
+// DFT_C of length C = 2^n > 2^11 is a four-step transform over C = C1 * C2:
+//   pass 1, per column i1: Y[t2][i1] = w_N^{i1 (c + Q t2)} sum_i2 x[i1 + C1 i2] w_N^{c C1 i2} w_C2^{i2 t2}
 //   pass 2, per row t2:    X[t2 + C2 t1] = sum_i1 Y[t2][i1] w_C1^{i1 t1}
-// Each pass stages a tile of whole columns/rows in LDS and runs an in-place
-// radix-2 decimation-in-frequency transform there; the bit-reversed result is
-// read back in natural order on the way out.  The coset twist w_N^{c i} is fused
-// into the loads of pass 1 (or of the single pass when C <= 2^11).
+// The coset twist w_N^{c i} = w_N^{c i1} w_N^{c C1 i2} is split: the column-constant
+// factor commutes with the column DFT and merges with the four-step twiddle
+// w_C^{i1 t2} into one post factor w_N^{i1 (c + Q t2)}.  Both factors come from
+// tables built once per shape (N words each), which every message and coset
+// shares, so they stay in L2/MALL while the data streams through HBM.
+// Goldilocks multiplication costs ~4x an addition on CDNA4, so the kernels are
+// VALU-bound: the in-LDS transforms use radix-8 register butterflies whose inner
+// rotations are the shifts w_8 = 2^120 and w_4 = 2^48 (2 has order 192 mod p),
+// leaving 7/8 of a generic multiplication per element per three radix-2 stages.
 #include "common.h"
 #include "gl.h"
 #include "ntt.h"
@@ -30,25 +35,46 @@ namespace whir {
 using gl::Ext;
 
 static constexpr int NTT_THREADS = 256;
-static constexpr int TILE_WORDS = 4096;   // u64 words of LDS per tile (32 KiB); a tile
-                                          // of one 2^11 Fp3 column (6144 words) still fits 64 KiB
-static constexpr int MAX_PASS_LOG = 11;   // largest DFT staged in one pass
+static constexpr int MAX_PASS_LOG = 11;   // largest DFT staged in one pass (a 2^11 Fp3
+                                          // column is 48 KiB, +16 KiB of twiddles)
+
+// u64 words of data per LDS tile: 4096 (32 KiB) unless WHIR_NTT_TILE_WORDS asks for
+// more and the device allows it (gfx950 has 160 KiB of LDS per workgroup).
+static int tile_words() {
+  static int w = [] {
+    int words = 4096;
+    if (const char* e = getenv("WHIR_NTT_TILE_WORDS")) words = atoi(e);
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
+      while (words > 1024 && (size_t)(words + 2048) * 8 > (size_t)lds) words /= 2;
+    }
+    int p = 1024;
+    while (p * 2 <= words) p *= 2;
+    return p;
+  }();
+  return w;
+}
+
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) WHIR_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
 
 // --------------------------------------------------------------------------
-// Twiddle tables: for n = 2^k, lo[j] = w_n^j (j < 2^h), hi[j] = w_n^{j 2^h},
-// h = ceil(k/2).  w_n^e = lo[e & (2^h-1)] * hi[e >> h].  Also a flat table
-// w_S^j, j < S/2, for the in-LDS transforms.
+// Twiddle tables.
 // --------------------------------------------------------------------------
 struct TwTable {
-  const uint64_t* lo = nullptr;
-  const uint64_t* hi = nullptr;
+  const uint64_t* lo = nullptr;  // w_n^j, j < 2^h
+  const uint64_t* hi = nullptr;  // w_n^(j 2^h)
   int h = 0;
 };
 
 namespace {
 std::mutex g_tw_mu;
-std::map<std::pair<int, int>, TwTable> g_tw;                    // (device, log n)
-std::map<std::pair<int, int>, const uint64_t*> g_flat;          // (device, log S)
+std::map<std::pair<int, int>, TwTable> g_tw;                                   // (device, log n)
+std::map<std::pair<int, int>, const uint64_t*> g_flat;                         // (device, log S)
+std::map<std::tuple<int, int, int, int>, const uint64_t*> g_shape;             // (device, kind, log N, log C)
 
 const uint64_t* upload_persistent(const std::vector<uint64_t>& v) {
   void* p = nullptr;
@@ -56,11 +82,15 @@ const uint64_t* upload_persistent(const std::vector<uint64_t>& v) {
   WHIR_HIP(hipMemcpy(p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
   return (const uint64_t*)p;
 }
+int current_device() {
+  int dev = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  return dev;
+}
 }  // namespace
 
 TwTable twiddles(int logn) {
-  int dev = 0;
-  WHIR_HIP(hipGetDevice(&dev));
+  int dev = current_device();
   std::lock_guard<std::mutex> g(g_tw_mu);
   auto key = std::make_pair(dev, logn);
   auto it = g_tw.find(key);
@@ -81,15 +111,14 @@ TwTable twiddles(int logn) {
   return t;
 }
 
+// w_S^j for every j < S (S <= 2^11): the in-LDS transform reads it from LDS.
 static const uint64_t* flat_twiddles(int logS) {
-  int dev = 0;
-  WHIR_HIP(hipGetDevice(&dev));
+  int dev = current_device();
   std::lock_guard<std::mutex> g(g_tw_mu);
   auto key = std::make_pair(dev, logS);
   auto it = g_flat.find(key);
   if (it != g_flat.end()) return it->second;
-  size_t half = logS ? (1ull << (logS - 1)) : 1;
-  std::vector<uint64_t> v(half);
+  std::vector<uint64_t> v((size_t)1 << logS);
   uint64_t w = gl::root_of_unity(1ull << logS);
   uint64_t x = 1;
   for (auto& e : v) { e = x; x = gl::mul(x, w); }
@@ -103,29 +132,157 @@ __device__ __forceinline__ uint64_t tw_at(const TwTable& t, uint64_t e) {
   return gl::mul(t.lo[e & mask], t.hi[e >> t.h]);
 }
 
+// kind 0: pre[c*C + i] = w_N^(c i)                        (c < Q, i < C)
+// kind 1: post[(c*C2 + t2)*C1 + i1] = w_N^(i1 (c + Q t2))  (pass-1 output order)
+__global__ void k_shape_table(uint64_t* out, TwTable twN, int kind, uint64_t N, int logC, int logC1) {
+  const uint64_t C = 1ull << logC;
+  for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < N; x += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t e;
+    if (kind == 0) {
+      e = (x >> logC) * (x & (C - 1));
+    } else {
+      const int logC2 = logC - logC1;
+      uint64_t i1 = x & ((1ull << logC1) - 1);
+      uint64_t row = x >> logC1;  // c*C2 + t2
+      uint64_t t2 = row & ((1ull << logC2) - 1), c = row >> logC2;
+      e = i1 * (c + (N >> logC) * t2);
+    }
+    out[x] = tw_at(twN, e & (N - 1));
+  }
+}
+
+static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipStream_t st) {
+  int dev = current_device();
+  {
+    std::lock_guard<std::mutex> g(g_tw_mu);
+    auto it = g_shape.find({dev, kind, logN, logC});
+    if (it != g_shape.end()) return it->second;
+  }
+  TwTable twN = twiddles(logN);
+  uint64_t N = 1ull << logN;
+  void* p = nullptr;
+  WHIR_HIP(hipMalloc(&p, N * 8));
+  unsigned blocks = (unsigned)std::min<uint64_t>((N + 255) / 256, 16384);
+  k_shape_table<<<blocks, 256, 0, st>>>((uint64_t*)p, twN, kind, N, logC, logC1);
+  WHIR_HIP(hipGetLastError());
+  WHIR_HIP(hipStreamSynchronize(st));
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  auto res = g_shape.emplace(std::make_tuple(dev, kind, logN, logC), (const uint64_t*)p);
+  if (!res.second) (void)hipFree(p);  // another thread built it first
+  return res.first->second;
+}
+
 // --------------------------------------------------------------------------
-// In-LDS radix-2 DIF over `cols` columns of length S = 2^logS stored
-// s[col*S + idx]; output X[k] lands at index bitrev(k).  tw[j] = w_S^j.
+// In-LDS DFT: `cols` columns of length S = 2^logS stored s[col*S + idx], natural
+// order in, bit-reversed order out (X[k] at index bitrev(k)).  tw[j] = w_S^j, j < S.
+//
+// A radix-R step (R = 2, 4, 8) on blocks of size S' takes, for each base < S'/R,
+// y_m = x[base + m S'/R], runs the R-point DIF in registers (which leaves
+// z_{bitrev(m)} in slot m), multiplies slot m by w_S'^(base * bitrev(m)) and
+// writes it back in place; that is exactly the state after log2(R) radix-2 DIF
+// stages, so the steps compose to the plain bit-reversed DIF.
 // --------------------------------------------------------------------------
 template <int D>
-__device__ void lds_dif(Ext<D>* s, const uint64_t* tw, int logS, int cols) {
+__device__ __forceinline__ Ext<D> rot(const Ext<D>& a, unsigned k) {
+  Ext<D> r;
+#pragma unroll
+  for (int d = 0; d < D; d++) r.c[d] = gl::mul_pow2(a.c[d], k);
+  return r;
+}
+
+template <int D>
+__device__ __forceinline__ void bfly(Ext<D>& a, Ext<D>& b) {
+  Ext<D> s = gl::eadd<D>(a, b), t = gl::esub<D>(a, b);
+  a = s;
+  b = t;
+}
+
+template <int D, int R>
+__device__ __forceinline__ void dif_regs(Ext<D> (&y)[R]) {
+  if constexpr (R == 2) {
+    bfly<D>(y[0], y[1]);
+  } else if constexpr (R == 4) {
+    bfly<D>(y[0], y[2]);
+    bfly<D>(y[1], y[3]);
+    y[3] = rot<D>(y[3], 48);  // w_4 = 2^48
+    bfly<D>(y[0], y[1]);
+    bfly<D>(y[2], y[3]);
+  } else {
+    bfly<D>(y[0], y[4]);
+    bfly<D>(y[1], y[5]);
+    bfly<D>(y[2], y[6]);
+    bfly<D>(y[3], y[7]);
+    y[5] = rot<D>(y[5], 120);  // w_8 = 2^120 = -2^24
+    y[6] = rot<D>(y[6], 48);   // w_8^2 = w_4
+    y[7] = rot<D>(y[7], 168);  // w_8^3 = 2^168
+    bfly<D>(y[0], y[2]);
+    bfly<D>(y[1], y[3]);
+    bfly<D>(y[4], y[6]);
+    bfly<D>(y[5], y[7]);
+    y[3] = rot<D>(y[3], 48);
+    y[7] = rot<D>(y[7], 48);
+    bfly<D>(y[0], y[1]);
+    bfly<D>(y[2], y[3]);
+    bfly<D>(y[4], y[5]);
+    bfly<D>(y[6], y[7]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ int brev_small(int m) {
+  if constexpr (R == 2) return m;
+  if constexpr (R == 4) return ((m & 1) << 1) | (m >> 1);
+  return ((m & 1) << 2) | (m & 2) | (m >> 2);
+}
+
+template <int D, int R, int LOGR>
+__device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS, int logSub, int cols) {
   const int S = 1 << logS;
-  const int nb = cols * (S >> 1);
-  for (int lh = logS - 1; lh >= 0; lh--) {
-    const int h = 1 << lh;
-    const int tstep = S >> (lh + 1);  // twiddle stride: w_{2h}^k = w_S^{k * S/(2h)}
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-      int col = q >> (logS - 1);
-      int p = q & ((S >> 1) - 1);
-      int k = p & (h - 1);
-      int i = ((p >> lh) << (lh + 1)) + k;
-      Ext<D>* base = s + col * S;
-      Ext<D> u = base[i], v = base[i + h];
-      base[i] = gl::eadd<D>(u, v);
-      base[i + h] = gl::escale<D>(gl::esub<D>(u, v), tw[k * tstep]);
+  const int stride = 1 << (logSub - LOGR);                 // S'/R
+  const int groups = cols << (logS - LOGR);                // cols * S / R
+  const int tw_shift = logS - logSub;                      // w_S' = w_S^(S/S')
+  for (int q = threadIdx.x; q < groups; q += blockDim.x) {
+    const int base = q & (stride - 1);
+    const int blk = q >> (logSub - LOGR);                  // col * (S/S') + block within column
+    Ext<D>* p = s + ((size_t)blk << logSub) + base;
+    Ext<D> y[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) y[m] = p[m * stride];
+    dif_regs<D, R>(y);
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      const int e = (base * brev_small<R>(m)) << tw_shift;
+      p[m * stride] = m == 0 ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
     }
+  }
+  (void)S;
+}
+
+template <int D>
+__device__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS, int cols) {
+  int logSub = logS;
+  while (logSub >= 3) {
+    dif_step<D, 8, 3>(s, tw, logS, logSub, cols);
+    __syncthreads();
+    logSub -= 3;
+  }
+  if (logSub == 2) {
+    dif_step<D, 4, 2>(s, tw, logS, logSub, cols);
+    __syncthreads();
+  } else if (logSub == 1) {
+    dif_step<D, 2, 1>(s, tw, logS, logSub, cols);
     __syncthreads();
   }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Decode the hardware block index so that consecutive logical blocks (the cosets
+// of one column tile, neighbouring tiles sharing cache lines) run on one XCD.
+__device__ __forceinline__ bool xcd_block(uint32_t nlogical, uint32_t& logical) {
+  constexpr uint32_t NX = 8;
+  const uint32_t per = (nlogical + NX - 1) / NX;
+  logical = (blockIdx.x % NX) * per + blockIdx.x / NX;
+  return logical < nlogical;
 }
 
 __device__ __forceinline__ int bitrev(int x, int bits) {
@@ -151,16 +308,17 @@ __device__ __forceinline__ Ext<D> coeff_at(const uint64_t* msg, size_t L, const 
 
 // --------------------------------------------------------------------------
 // Single pass: C = 2^logC <= 2^11.  Each block transforms G sequences.
+// pre[c*C + i] = w_N^(c i).
 // --------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, TwTable twN, const uint64_t* twS,
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint64_t* pre, const uint64_t* twS,
                                                             int G) {
   extern __shared__ uint64_t smem[];
   const int logC = a.logC;
   const int C = 1 << logC;
   Ext<D>* s = (Ext<D>*)smem;
   uint64_t* tws = smem + (size_t)G * C * D;
-  for (int k = threadIdx.x; k < (C >> 1); k += blockDim.x) tws[k] = twS[k];
+  for (int k = threadIdx.x; k < C; k += blockDim.x) tws[k] = twS[k];
   const size_t nseq = (size_t)a.M * a.Q;
   const size_t seq0 = (size_t)blockIdx.x * G;
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
@@ -171,13 +329,12 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, TwTable tw
       size_t j = seq / a.Q, c = seq % a.Q;
       v = coeff_at<D>(a.msgs + j * a.msg_stride * D, a.L, a.masks ? a.masks + j * a.mask_len * D : nullptr,
                       a.mask_len, i);
-      uint64_t ex = ((uint64_t)c * (uint64_t)i) & (a.N - 1);
-      if (ex) v = gl::escale<D>(v, tw_at(twN, ex));
+      if (c) v = gl::escale<D>(v, pre[c * C + i]);
     }
     s[e] = v;
   }
   __syncthreads();
-  lds_dif<D>(s, tws, logC, G);
+  lds_dft<D>(s, tws, logC, G);
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
     int g = e >> logC, t = e & (C - 1);
     size_t seq = seq0 + g;
@@ -196,42 +353,44 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, TwTable tw
 // Writes Y (same shape as the codeword) at [j][c*C + t2*C1 + i1].
 // --------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, TwTable twN, TwTable twC,
-                                                           const uint64_t* twS, uint64_t* Y, int logC1, int b) {
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint64_t* pre, const uint64_t* post,
+                                                           const uint64_t* twS, uint64_t* Y, int logC1, int logb) {
   extern __shared__ uint64_t smem[];
   const int logC2 = a.logC - logC1;
-  const int C1 = 1 << logC1, C2 = 1 << logC2;
+  const int C1 = 1 << logC1, C2 = 1 << logC2, b = 1 << logb;
   const size_t C = (size_t)1 << a.logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + (size_t)b * C2 * D;
-  for (int k = threadIdx.x; k < (C2 >> 1); k += blockDim.x) tws[k] = twS[k];
-  const int i1_0 = blockIdx.x * b;
-  const size_t c = blockIdx.y, j = blockIdx.z;
+  uint64_t* tws = smem + ((size_t)C2 << logb) * D;
+  for (int k = threadIdx.x; k < C2; k += blockDim.x) tws[k] = twS[k];
+  // logical block = (j * ntile + tile) * Q + c: the Q cosets of a tile are adjacent
+  const uint32_t ntile = (uint32_t)(C1 >> logb);
+  uint32_t lb;
+  if (!xcd_block(ntile * (uint32_t)a.Q * (uint32_t)a.M, lb)) return;
+  const size_t c = lb % a.Q, j = (lb / a.Q) / ntile;
+  const int i1_0 = (int)((lb / a.Q) % ntile) << logb;
   const uint64_t* msg = a.msgs + j * a.msg_stride * D;
   const uint64_t* mask = a.masks ? a.masks + j * a.mask_len * D : nullptr;
-  const int tile = b * C2;
-  {
-    for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-      int col = e % b, i2 = e / b;
-      size_t i = (size_t)(i1_0 + col) + (size_t)C1 * i2;
-      Ext<D> v = coeff_at<D>(msg, a.L, mask, a.mask_len, i);
-      uint64_t ex = ((uint64_t)c * (uint64_t)i) & (a.N - 1);
-      if (ex) v = gl::escale<D>(v, tw_at(twN, ex));
-      s[col * C2 + i2] = v;
-    }
-    __syncthreads();
-    lds_dif<D>(s, tws, logC2, b);
-    uint64_t* yj = Y + (j * a.N + c * C) * D;
-    for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-      int col = e % b, t2 = e / b;
-      int i1 = i1_0 + col;
-      Ext<D> v = s[col * C2 + bitrev(t2, logC2)];
-      uint64_t ex = (uint64_t)i1 * (uint64_t)t2;
-      if (ex) v = gl::escale<D>(v, tw_at(twC, ex));
-      uint64_t* o = yj + ((size_t)t2 * C1 + i1) * D;
+  const int tile = C2 << logb;
+  const uint64_t* prec = pre + c * C;  // w_N^(c i) for i < C
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
+    int col = e & (b - 1), i2 = e >> logb;
+    size_t i = (size_t)(i1_0 + col) + ((size_t)i2 << logC1);
+    Ext<D> v = coeff_at<D>(msg, a.L, mask, a.mask_len, i);
+    if (c) v = gl::escale<D>(v, prec[(size_t)i2 << logC1]);  // row part w_N^(c C1 i2)
+    s[(col << logC2) + i2] = v;
+  }
+  __syncthreads();
+  lds_dft<D>(s, tws, logC2, b);
+  uint64_t* yj = Y + (j * a.N + c * C) * D;
+  const uint64_t* postc = post + c * C;
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
+    int col = e & (b - 1), t2 = e >> logb;
+    int i1 = i1_0 + col;
+    size_t o_idx = ((size_t)t2 << logC1) + i1;
+    Ext<D> v = gl::escale<D>(s[(col << logC2) + bitrev(t2, logC2)], postc[o_idx]);
+    uint64_t* o = yj + o_idx * D;
 #pragma unroll
-      for (int d = 0; d < D; d++) o[d] = v.c[d];
-    }
+    for (int d = 0; d < D; d++) o[d] = v.c[d];
   }
 }
 
@@ -240,18 +399,22 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, TwTable twN
 // --------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint64_t* twS, const uint64_t* Y,
-                                                           int logC1, int b) {
+                                                           int logC1, int logb) {
   extern __shared__ uint64_t smem[];
   const int logC2 = a.logC - logC1;
-  const int C1 = 1 << logC1, C2 = 1 << logC2;
+  const int C1 = 1 << logC1, C2 = 1 << logC2, b = 1 << logb;
   const size_t C = (size_t)1 << a.logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + (size_t)b * C1 * D;
-  for (int k = threadIdx.x; k < (C1 >> 1); k += blockDim.x) tws[k] = twS[k];
-  const int t2_0 = blockIdx.x * b;
-  const size_t c = blockIdx.y, j = blockIdx.z;
+  uint64_t* tws = smem + ((size_t)C1 << logb) * D;
+  for (int k = threadIdx.x; k < C1; k += blockDim.x) tws[k] = twS[k];
+  // logical block = (j * Q + c) * ntile + tile: neighbouring row tiles are adjacent
+  const uint32_t ntile = (uint32_t)(C2 >> logb);
+  uint32_t lb;
+  if (!xcd_block(ntile * (uint32_t)a.Q * (uint32_t)a.M, lb)) return;
+  const int t2_0 = (int)(lb % ntile) << logb;
+  const size_t c = (lb / ntile) % a.Q, j = (lb / ntile) / a.Q;
   const uint64_t* yj = Y + (j * a.N + c * C) * D;
-  const int tile = b * C1;
+  const int tile = C1 << logb;
   for (int e = threadIdx.x; e < tile; e += blockDim.x) {
     int row = e >> logC1, i1 = e & (C1 - 1);
     const uint64_t* src = yj + ((size_t)(t2_0 + row) * C1 + i1) * D;
@@ -261,15 +424,16 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
     s[e] = v;
   }
   __syncthreads();
-  lds_dif<D>(s, tws, logC1, b);
+  lds_dft<D>(s, tws, logC1, b);
   uint64_t* oj = a.out + (j * a.N + c * C) * D;
   for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int row = e % b, t1 = e / b;
-    Ext<D> v = s[row * C1 + bitrev(t1, logC1)];
-    uint64_t* o = oj + ((size_t)(t2_0 + row) + (size_t)C2 * t1) * D;
+    int row = e & (b - 1), t1 = e >> logb;
+    Ext<D> v = s[(row << logC1) + bitrev(t1, logC1)];
+    uint64_t* o = oj + ((size_t)(t2_0 + row) + ((size_t)t1 << logC2)) * D;
 #pragma unroll
     for (int d = 0; d < D; d++) o[d] = v.c[d];
   }
+  (void)C2;
 }
 
 // Column-major [M][N] -> row-major [N][M] (host-facing layout of interleaved_encode).
@@ -300,48 +464,67 @@ static int log2_exact(uint64_t n) {
   return (1ull << k) == n ? k : -1;
 }
 
-static int floor_pow2(int x) {
-  int p = 1;
-  while (p * 2 <= x) p *= 2;
-  return p;
+// Hardware blocks for `nlogical` logical blocks decoded by xcd_block.
+static unsigned xcd_grid(uint64_t nlogical) {
+  WHIR_REQUIRE(nlogical < (1ull << 31), "interleaved_encode: too many blocks");
+  uint64_t per = (nlogical + 7) / 8;
+  return (unsigned)(per * 8);
+}
+
+static int floor_log2(int x) {
+  int k = 0;
+  while ((2 << k) <= x) k++;
+  return k;
+}
+
+static const char* dname(const char* base, int D) {
+  static std::mutex mu;
+  static std::map<std::pair<std::string, int>, std::string> names;
+  std::lock_guard<std::mutex> g(mu);
+  auto& s = names[{base, D}];
+  if (s.empty()) s = std::string(base) + "<" + std::to_string(D) + ">";
+  return s.c_str();
 }
 
 template <int D>
 static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
-  TwTable twN = twiddles(log2_exact(a.N));
+  const int logN = log2_exact(a.N);
   const int logC = a.logC;
   const size_t nseq = (size_t)a.M * a.Q;
+  const uint64_t* pre = shape_table(0, logN, logC, 0, st);
+  const int TW = tile_words();
   if (logC <= MAX_PASS_LOG) {
     int C = 1 << logC;
-    int G = std::max(1, floor_pow2(TILE_WORDS / (C * D)));
+    int G = 1 << std::max(0, floor_log2(std::max(1, TW / (C * D))));
     const uint64_t* twS = flat_twiddles(logC);
-    size_t smem = ((size_t)G * C * D + std::max(1, C / 2)) * 8;
+    size_t smem = ((size_t)G * C * D + C) * 8;
     unsigned blocks = (unsigned)((nseq + G - 1) / G);
-    ProfScope prof("ntt_single", st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-    k_ntt_single<D><<<blocks, NTT_THREADS, smem, st>>>(a, twN, twS, G);
+    allow_lds(k_ntt_single<D>, smem);
+    ProfScope prof(dname("ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
+    k_ntt_single<D><<<blocks, NTT_THREADS, smem, st>>>(a, pre, twS, G);
     WHIR_HIP(hipGetLastError());
     return;
   }
   WHIR_REQUIRE(logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
   const int logC1 = (logC + 1) / 2, logC2 = logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
-  TwTable twC = twiddles(logC);
-  // pass 1
+  const uint64_t* post = shape_table(1, logN, logC, logC1, st);
   {
-    int b = std::min(C1, std::max(1, floor_pow2(TILE_WORDS / (C2 * D))));
-    size_t smem = ((size_t)b * C2 * D + C2 / 2) * 8;
-    dim3 grid(C1 / b, a.Q, a.M);
-    ProfScope prof("ntt_pass1", st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-    k_ntt_pass1<D><<<grid, NTT_THREADS, smem, st>>>(a, twN, twC, flat_twiddles(logC2), scratch, logC1, b);
+    int logb = std::min(logC1, floor_log2(std::max(1, TW / (C2 * D))));
+    size_t smem = (((size_t)C2 << logb) * D + C2) * 8;
+    dim3 grid(xcd_grid((uint64_t)(C1 >> logb) * a.Q * a.M));
+    allow_lds(k_ntt_pass1<D>, smem);
+    ProfScope prof(dname("ntt_pass1", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
+    k_ntt_pass1<D><<<grid, NTT_THREADS, smem, st>>>(a, pre, post, flat_twiddles(logC2), scratch, logC1, logb);
     WHIR_HIP(hipGetLastError());
   }
-  // pass 2
   {
-    int b = std::min(C2, std::max(1, floor_pow2(TILE_WORDS / (C1 * D))));
-    size_t smem = ((size_t)b * C1 * D + C1 / 2) * 8;
-    dim3 grid(C2 / b, a.Q, a.M);
-    ProfScope prof("ntt_pass2", st, 16.0 * D * (double)a.M * a.N);
-    k_ntt_pass2<D><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, b);
+    int logb = std::min(logC2, floor_log2(std::max(1, TW / (C1 * D))));
+    size_t smem = (((size_t)C1 << logb) * D + C1) * 8;
+    dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
+    allow_lds(k_ntt_pass2<D>, smem);
+    ProfScope prof(dname("ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
+    k_ntt_pass2<D><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
     WHIR_HIP(hipGetLastError());
   }
 }
@@ -354,8 +537,7 @@ size_t rs_scratch_words(int D, uint64_t M, uint64_t N, uint64_t C) {
 void rs_encode_cols(int D, RsArgs a, uint64_t* scratch, hipStream_t st) {
   WHIR_REQUIRE(a.M > 0, "rs_encode: no messages");
   WHIR_REQUIRE(log2_exact(a.N) >= 0, "rs_encode: the device path needs a power-of-two codeword length");
-  WHIR_REQUIRE(log2_exact(1ull << a.logC) >= 0 && ((a.N >> a.logC) << a.logC) == a.N,
-               "rs_encode: coset size must divide the codeword length");
+  WHIR_REQUIRE(((a.N >> a.logC) << a.logC) == a.N, "rs_encode: coset size must divide the codeword length");
   switch (D) {
     case 1: launch<1>(a, scratch, st); break;
     case 2: launch<2>(a, scratch, st); break;
