@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--num-variables", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run one device-to-device copy of this many bytes (a known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -78,7 +81,12 @@ def main():
     pid = hashlib.sha3_512(json.dumps(desc, sort_keys=True).encode()).digest()
     sid = hashlib.sha3_256(b"whir_amd bench").digest()
 
-    W.profile_enable(True)
+    if args.calibrate_copy:
+        src = W.DeviceBuffer(args.calibrate_copy)
+        dst = W.DeviceBuffer(args.calibrate_copy)
+        W._call("whir_memcpy_d2d", dst.ptr, src.ptr, args.calibrate_copy)
+        W.synchronize()
+        del src, dst
 
     def step():
         t = GP.Transcript(pid, sid)
@@ -89,7 +97,6 @@ def main():
     for _ in range(args.warmup):
         step()
     W.synchronize()
-    W.profile_reset()
     t0 = time.perf_counter()
     last = None
     for _ in range(args.steps):
@@ -97,6 +104,15 @@ def main():
     W.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
+
+    # Per-kernel times: the same steps again with a HIP event pair around every
+    # launch, recorded on the stream the kernel runs on (kept out of the timed
+    # region above, since the events add host work between launches).
+    W.profile_enable(True)
+    W.profile_reset()
+    for _ in range(args.steps):
+        step()
+    W.synchronize()
     prof = W.profile_report()
     W.profile_enable(False)
 
