@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Per-launch HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from rocprofv3 --pmc
+# passes over this workload at d=24; matched to the dominant kernel by name.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_d24.json")
 
 
 def parse():
@@ -123,8 +126,13 @@ def main():
         avg_ms = dom["total_ms"] / dom["launches"]
         bytes_per_launch = dom["bytes"] / dom["launches"]
         gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        traffic = None
+        if nv == 24 and os.path.exists(TRAFFIC_FILE):
+            entry = json.load(open(TRAFFIC_FILE))["kernels"].get(dom_name)
+            if entry:
+                traffic = int(entry["traffic_bytes_per_launch"])
         roofline = {"kernel": dom_name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
     verified = None

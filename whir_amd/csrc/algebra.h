@@ -27,6 +27,18 @@ void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const ui
 void geometric_accumulate_dev(int D, uint64_t* acc, size_t n, const uint64_t* scalars, const uint64_t* points,
                               size_t m, hipStream_t st);
 
+// Both of the above, and any number of each, in one pass:
+//   acc[i] = (accumulate ? acc[i] : 0) + sum_p eq_scalars_p eq(eq_points_p, i) + sum_j scalars_j points_j^i
+// for i < n (n = 2^nv when np > 0).  Points and scalars are host arrays.
+void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_t* eq_points,
+                  const uint64_t* eq_scalars, size_t np, size_t nv, const uint64_t* scalars, const uint64_t* points,
+                  size_t m, hipStream_t st);
+
+// mixed_univariate_evaluate at m host points x[0..m) in shared passes; results to
+// dev_out[0..m D).
+void univariate_eval_many_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, size_t m,
+                              uint64_t* dev_out, hipStream_t st);
+
 // mixed_univariate_evaluate (src/algebra/mod.rs:105-129): sum_i c_i x^i with c in the
 // base field (Ds = 1) or in the extension (Ds = D); result to dev_out[0..D).
 void univariate_eval_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, uint64_t* dev_out,

@@ -128,9 +128,11 @@ static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* ou
   size_t half = next_pow2(n) >> 1, nhi = n - half;
   unsigned g = grid_for(half);
   DevBuf parts((size_t)g * 2 * D * 8, s);
-  ProfScope prof("sumcheck_poly", s, 16.0 * D * (double)n);
-  k_sc_poly<D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
-  WHIR_HIP(hipGetLastError());
+  {
+    ProfScope prof(kernel_name("k_sc_poly", D), s, 16.0 * D * (double)n);
+    k_sc_poly<D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
+    WHIR_HIP(hipGetLastError());
+  }
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
 }
 
@@ -163,7 +165,7 @@ static size_t fold_t(uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) {
   size_t half = next_pow2(n) >> 1, nhi = n - half;
   Ext<D> ww = host_ext<D>(w);
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
-  ProfScope prof("fold", s, 8.0 * D * ((double)n + (double)half));
+  ProfScope prof(kernel_name("k_fold", D), s, 8.0 * D * ((double)n + (double)half));
   k_fold<D><<<grid_for(half), RT, 0, s>>>(v, half, nhi, ww, om);
   WHIR_HIP(hipGetLastError());
   return half;
@@ -218,9 +220,11 @@ static size_t fold_poly_t(uint64_t* a, uint64_t* b, size_t n, const uint64_t* w,
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
   unsigned g = grid_for(h >> 1);
   DevBuf parts((size_t)g * 2 * D * 8, s);
-  ProfScope prof("fold_sumcheck_poly", s, 8.0 * D * (2.0 * n + 2.0 * h));
-  k_fold_poly<D><<<g, RT, 0, s>>>(a, b, h, nhi, ww, om, parts.as<uint64_t>());
-  WHIR_HIP(hipGetLastError());
+  {
+    ProfScope prof(kernel_name("k_fold_poly", D), s, 8.0 * D * (2.0 * n + 2.0 * h));
+    k_fold_poly<D><<<g, RT, 0, s>>>(a, b, h, nhi, ww, om, parts.as<uint64_t>());
+    WHIR_HIP(hipGetLastError());
+  }
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
   return h;
 }
@@ -235,242 +239,347 @@ size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64
   throw Error("fold: extension degree must be 1, 2 or 3");
 }
 
-// ------------------------------------------------------------------ eval_eq
-// eq(z, x) = eq_hi(x >> k2) * eq_lo(x & (2^k2 - 1)); the tables are built on the
-// host (2^k1 + 2^k2 entries) and the scalar is folded into eq_hi.
-template <int D>
-__global__ __launch_bounds__(RT) void k_eval_eq(uint64_t* acc, size_t n, const uint64_t* hi, const uint64_t* lo,
-                                                int k2) {
-  const size_t mask = ((size_t)1 << k2) - 1;
-  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n; x += (size_t)gridDim.x * blockDim.x) {
-    Ext<D> e = gl::emul<D>(ld<D>(hi, x >> k2), ld<D>(lo, x & mask));
-    st<D>(acc, x, gl::eadd<D>(ld<D>(acc, x), e));
-  }
-}
+// ------------------------------------------------------------------ covector terms
+// covector_dev adds (or writes) any mix of
+//   eq terms     s_p eq(z_p, i)      eval_eq (src/algebra/multilinear.rs:170-190)
+//   power terms  t_j x_j^i           geometric_accumulate (src/algebra/mod.rs:154-177)
+// for every i < n in one pass over HBM.  Element layout: block b covers
+// [b S, (b+1) S) with S = RT K; lane t owns i = b S + k RT + t (k < K), so loads
+// and stores are coalesced.  k_cov_tables builds every table on the device:
+//   eq:    hi_p[h] = s_p prod_{v < k1} (h_v ? z_v : 1 - z_v) over the first k1
+//          variables (h_0 = MSB), lo_p[l] the same over the last k2; eq = hi lo.
+//   power: T0[j nb + b] = t_j x_j^(b S), T1[j RT + t] = x_j^t, X[j] = x_j^RT.
+//          Base-field points (upper coefficients zero; the STIR in-domain points)
+//          keep T1 and X in the base field, so a step costs D base products
+//          instead of an extension product.
+struct CovPlan {
+  size_t np, nv, k1, k2, nhi, nlo;        // eq terms
+  size_t m_e, m_b, nb, S;                 // power terms: extension points, then base points
+  size_t in_z, in_s, in_x, in_t;          // uploaded inputs (word offsets)
+  size_t hi, lo, T0, T1e, Xe, T1b, Xb;    // tables (word offsets)
+  size_t words, entries;
+};
 
-// Host table: t[x] = s * prod_{i<k} (x_i ? z_i : 1 - z_i), x_0 = MSB of x.
 template <int D>
-static std::vector<uint64_t> eq_table(const uint64_t* z, size_t k, Ext<D> s) {
-  std::vector<uint64_t> t(((size_t)1 << k) * D);
-  std::vector<Ext<D>> cur{s};
-  for (size_t i = 0; i < k; i++) {
-    Ext<D> zi = host_ext<D>(z + i * D);
-    Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), zi);
-    std::vector<Ext<D>> nxt(cur.size() * 2);
-    for (size_t x = 0; x < cur.size(); x++) {
-      nxt[2 * x] = gl::emul<D>(cur[x], om);
-      nxt[2 * x + 1] = gl::emul<D>(cur[x], zi);
+__global__ void k_cov_tables(uint64_t* buf, CovPlan L) {
+  size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= L.entries) return;
+  const uint64_t* z = buf + L.in_z;
+  const uint64_t* sc = buf + L.in_s;
+  const uint64_t* xs = buf + L.in_x;
+  const uint64_t* ts = buf + L.in_t;
+  const size_t m = L.m_e + L.m_b;
+  const size_t sec[7] = {L.np * L.nhi, L.np * L.nlo, m * L.nb, L.m_e * RT, L.m_e, L.m_b * RT, L.m_b};
+  if (e < sec[0]) {
+    const size_t p = e / L.nhi, h = e % L.nhi;
+    Ext<D> v = ld<D>(sc, p);
+    for (size_t q = 0; q < L.k1; q++) {
+      Ext<D> zq = ld<D>(z, p * L.nv + q);
+      v = gl::emul<D>(v, ((h >> (L.k1 - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
     }
-    cur.swap(nxt);
+    st<D>(buf + L.hi, e, v);
+    return;
   }
-  for (size_t x = 0; x < cur.size(); x++)
-    for (int d = 0; d < D; d++) t[x * D + d] = cur[x].c[d];
-  return t;
-}
-
-template <int D>
-static void eval_eq_t(uint64_t* acc, const uint64_t* z, size_t k, const uint64_t* scalar, hipStream_t s) {
-  size_t k1 = k / 2, k2 = k - k1;
-  auto hi = eq_table<D>(z, k1, host_ext<D>(scalar));
-  auto lo = eq_table<D>(z + k1 * D, k2, gl::ext_base<D>(1));
-  DevBuf dh(hi.size() * 8, s), dl(lo.size() * 8, s);
-  upload_async(dh.ptr, hi.data(), hi.size() * 8, s);
-  upload_async(dl.ptr, lo.data(), lo.size() * 8, s);
-  size_t n = (size_t)1 << k;
-  ProfScope prof("eval_eq", s, 16.0 * D * (double)n);
-  k_eval_eq<D><<<grid_for(n), RT, 0, s>>>(acc, n, dh.as<uint64_t>(), dl.as<uint64_t>(), (int)k2);
-  WHIR_HIP(hipGetLastError());
-}
-
-void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t s) {
-  switch (D) {
-    case 1: eval_eq_t<1>(acc, point, k, scalar, s); break;
-    case 2: eval_eq_t<2>(acc, point, k, scalar, s); break;
-    case 3: eval_eq_t<3>(acc, point, k, scalar, s); break;
-    default: throw Error("eval_eq: extension degree must be 1, 2 or 3");
+  e -= sec[0];
+  if (e < sec[1]) {
+    const size_t p = e / L.nlo, l = e % L.nlo;
+    Ext<D> v = gl::ext_base<D>(1);
+    for (size_t q = 0; q < L.k2; q++) {
+      Ext<D> zq = ld<D>(z, p * L.nv + L.k1 + q);
+      v = gl::emul<D>(v, ((l >> (L.k2 - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
+    }
+    st<D>(buf + L.lo, e, v);
+    return;
   }
+  e -= sec[1];
+  if (e < sec[2]) {
+    const size_t j = e / L.nb, b = e % L.nb;
+    const uint64_t ex = (uint64_t)b * L.S;
+    Ext<D> v = j < L.m_e ? gl::emul<D>(ld<D>(ts, j), gl::epow<D>(ld<D>(xs, j), ex))
+                         : gl::escale<D>(ld<D>(ts, j), gl::pow(xs[j * D], ex));
+    st<D>(buf + L.T0, e, v);
+    return;
+  }
+  e -= sec[2];
+  if (e < sec[3]) {
+    st<D>(buf + L.T1e, e, gl::epow<D>(ld<D>(xs, e / RT), e % RT));
+    return;
+  }
+  e -= sec[3];
+  if (e < sec[4]) {
+    st<D>(buf + L.Xe, e, gl::epow<D>(ld<D>(xs, e), RT));
+    return;
+  }
+  e -= sec[4];
+  if (e < sec[5]) {
+    buf[L.T1b + e] = gl::pow(xs[(L.m_e + e / RT) * D], e % RT);
+    return;
+  }
+  e -= sec[5];
+  buf[L.Xb + e] = gl::pow(xs[(L.m_e + e) * D], RT);
 }
 
-// ------------------------------------------------------------------ geometric accumulate
-// Block b covers [b*RT*K, (b+1)*RT*K); lane t owns i = b*RT*K + t + k*RT.
-// For each point j: p = s_j * x_j^(b*RT*K) * x_j^t, then acc_k += p, p *= x_j^RT.
-// xb[j*nb + b] = s_j x_j^(b RT K), xt[j*RT + t] = x_j^t, xs[j] = x_j^RT (host tables).
-static constexpr int GA_K = 16;
-
-// Points are split into extension points (first m_e of xb; tables xt_e, xs_e in
-// the extension) and base-field points (next m_b; tables xt_b, xs_b in the base
-// field, stepped with D base multiplications instead of a full extension product).
-// The in-domain STIR points are all base-field elements.
-template <int D>
-__global__ __launch_bounds__(RT) void k_geom(uint64_t* acc, size_t n, const uint64_t* xb, const uint64_t* xt_e,
-                                             const uint64_t* xs_e, const uint64_t* xt_b, const uint64_t* xs_b,
-                                             size_t m_e, size_t m_b, size_t nb) {
+template <int D, int K>
+__global__ __launch_bounds__(RT) void k_cov(uint64_t* acc, size_t n, int accumulate, const uint64_t* buf,
+                                            CovPlan L) {
   const size_t b = blockIdx.x;
-  const size_t i0 = b * (size_t)RT * GA_K + threadIdx.x;
-  Ext<D> sum[GA_K];
+  const size_t i0 = b * (size_t)RT * K + threadIdx.x;
+  Ext<D> sum[K];
 #pragma unroll
-  for (int k = 0; k < GA_K; k++) sum[k] = gl::ext_zero<D>();
-  for (size_t j = 0; j < m_e; j++) {
-    Ext<D> p = gl::emul<D>(ld<D>(xb, j * nb + b), ld<D>(xt_e, j * RT + threadIdx.x));
-    Ext<D> step = ld<D>(xs_e, j);
+  for (int k = 0; k < K; k++) sum[k] = gl::ext_zero<D>();
+  if (L.np) {
+    const size_t mask = L.nlo - 1;
+    for (size_t p = 0; p < L.np; p++) {
+      const uint64_t* hi = buf + L.hi + p * L.nhi * D;
+      const uint64_t* lo = buf + L.lo + p * L.nlo * D;
 #pragma unroll
-    for (int k = 0; k < GA_K; k++) {
-      sum[k] = gl::eadd<D>(sum[k], p);
-      p = gl::emul<D>(p, step);
+      for (int k = 0; k < K; k++) {
+        const size_t i = i0 + (size_t)k * RT;
+        if (i < n) sum[k] = gl::eadd<D>(sum[k], gl::emul<D>(ld<D>(hi, i >> L.k2), ld<D>(lo, i & mask)));
+      }
     }
   }
-  for (size_t j = 0; j < m_b; j++) {
-    Ext<D> p = gl::escale<D>(ld<D>(xb, (m_e + j) * nb + b), xt_b[j * RT + threadIdx.x]);
-    const uint64_t step = xs_b[j];
+  for (size_t j = 0; j < L.m_e; j++) {
+    Ext<D> p = gl::emul<D>(ld<D>(buf + L.T0, j * L.nb + b), ld<D>(buf + L.T1e, j * RT + threadIdx.x));
+    const Ext<D> step = ld<D>(buf + L.Xe, j);
 #pragma unroll
-    for (int k = 0; k < GA_K; k++) {
+    for (int k = 0; k < K; k++) {
       sum[k] = gl::eadd<D>(sum[k], p);
-      p = gl::escale<D>(p, step);
+      if (k + 1 < K) p = gl::emul<D>(p, step);
+    }
+  }
+  for (size_t j = 0; j < L.m_b; j++) {
+    Ext<D> p = gl::escale<D>(ld<D>(buf + L.T0, (L.m_e + j) * L.nb + b), buf[L.T1b + j * RT + threadIdx.x]);
+    const uint64_t step = buf[L.Xb + j];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      sum[k] = gl::eadd<D>(sum[k], p);
+      if (k + 1 < K) p = gl::escale<D>(p, step);
     }
   }
 #pragma unroll
-  for (int k = 0; k < GA_K; k++) {
-    size_t i = i0 + (size_t)k * RT;
-    if (i < n) st<D>(acc, i, gl::eadd<D>(ld<D>(acc, i), sum[k]));
+  for (int k = 0; k < K; k++) {
+    const size_t i = i0 + (size_t)k * RT;
+    if (i < n) st<D>(acc, i, accumulate ? gl::eadd<D>(ld<D>(acc, i), sum[k]) : sum[k]);
   }
 }
 
+// Lanes per element run: the largest K <= 16 that still gives >= 1024 blocks.
+static int cov_k(size_t n) {
+  int K = 16;
+  while (K > 1 && (n + (size_t)RT * K - 1) / ((size_t)RT * K) < 1024) K >>= 1;
+  return K;
+}
+
 template <int D>
-static void geom_t(uint64_t* acc, size_t n, const uint64_t* scalars, const uint64_t* points, size_t m,
-                   hipStream_t s) {
-  if (!n || !m) return;
-  size_t span = (size_t)RT * GA_K;
-  size_t nb = (n + span - 1) / span;
+static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t* eq_points,
+                       const uint64_t* eq_scalars, size_t np, size_t nv, const uint64_t* scalars,
+                       const uint64_t* points, size_t m, hipStream_t s) {
+  if (!n) return;
+  WHIR_REQUIRE(!np || n == ((size_t)1 << nv), "eval_eq: the covector length must be 2^(number of variables)");
+  if (!np && !m) {
+    if (!accumulate) fill_zero(acc, n * D * 8, s);
+    return;
+  }
+  const int K = cov_k(n);
+  CovPlan L{};
+  L.np = np;
+  L.nv = np ? nv : 0;
+  L.k1 = L.nv / 2;
+  L.k2 = L.nv - L.k1;
+  L.nhi = (size_t)1 << L.k1;
+  L.nlo = (size_t)1 << L.k2;
+  L.S = (size_t)RT * K;
+  L.nb = (n + L.S - 1) / L.S;
   // Extension points first, then points that lie in the base field.
   std::vector<size_t> order_e, order_b;
   for (size_t j = 0; j < m; j++) {
-    bool base = true;
+    bool base = D > 1;
     for (int d = 1; d < D; d++) base = base && points[j * D + d] == 0;
-    (base && D > 1 ? order_b : order_e).push_back(j);
+    (base ? order_b : order_e).push_back(j);
   }
-  const size_t m_e = order_e.size(), m_b = order_b.size();
-  std::vector<uint64_t> xb(m * nb * D), xt_e(std::max<size_t>(m_e, 1) * RT * D), xs_e(std::max<size_t>(m_e, 1) * D);
-  std::vector<uint64_t> xt_b(std::max<size_t>(m_b, 1) * RT), xs_b(std::max<size_t>(m_b, 1));
-  size_t slot = 0;
-  for (size_t q = 0; q < m_e + m_b; q++, slot++) {
-    const bool is_base = q >= m_e;
-    const size_t j = is_base ? order_b[q - m_e] : order_e[q];
-    Ext<D> x = host_ext<D>(points + j * D);
-    Ext<D> sj = host_ext<D>(scalars + j * D);
-    Ext<D> xspan = gl::epow<D>(x, span);
-    Ext<D> cur = sj;
-    for (size_t b = 0; b < nb; b++) {
-      for (int d = 0; d < D; d++) xb[(slot * nb + b) * D + d] = cur.c[d];
-      cur = gl::emul<D>(cur, xspan);
-    }
-    Ext<D> pw = gl::ext_base<D>(1);
-    for (int t = 0; t < RT; t++) {
-      if (is_base) xt_b[(q - m_e) * RT + t] = pw.c[0];
-      else
-        for (int d = 0; d < D; d++) xt_e[(q * RT + t) * D + d] = pw.c[d];
-      pw = gl::emul<D>(pw, x);
-    }
-    if (is_base) xs_b[q - m_e] = pw.c[0];  // x^RT
-    else
-      for (int d = 0; d < D; d++) xs_e[q * D + d] = pw.c[d];
-  }
-  DevBuf db(xb.size() * 8, s), dte(xt_e.size() * 8, s), dse(xs_e.size() * 8, s), dtb(xt_b.size() * 8, s),
-      dsb(xs_b.size() * 8, s);
-  upload_async(db.ptr, xb.data(), xb.size() * 8, s);
-  upload_async(dte.ptr, xt_e.data(), xt_e.size() * 8, s);
-  upload_async(dse.ptr, xs_e.data(), xs_e.size() * 8, s);
-  upload_async(dtb.ptr, xt_b.data(), xt_b.size() * 8, s);
-  upload_async(dsb.ptr, xs_b.data(), xs_b.size() * 8, s);
-  ProfScope prof("geometric_accumulate", s, 16.0 * D * (double)n);
-  k_geom<D><<<(unsigned)nb, RT, 0, s>>>(acc, n, db.as<uint64_t>(), dte.as<uint64_t>(), dse.as<uint64_t>(),
-                                        dtb.as<uint64_t>(), dsb.as<uint64_t>(), m_e, m_b, nb);
+  L.m_e = order_e.size();
+  L.m_b = order_b.size();
+  std::vector<uint64_t> in;
+  L.in_z = 0;
+  if (np) in.insert(in.end(), eq_points, eq_points + np * L.nv * D);
+  L.in_s = in.size();
+  if (np) in.insert(in.end(), eq_scalars, eq_scalars + np * D);
+  L.in_x = in.size();
+  for (size_t j : order_e) in.insert(in.end(), points + j * D, points + (j + 1) * D);
+  for (size_t j : order_b) in.insert(in.end(), points + j * D, points + (j + 1) * D);
+  L.in_t = in.size();
+  for (size_t j : order_e) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
+  for (size_t j : order_b) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
+  size_t w = in.size();
+  L.hi = w;  w += np * L.nhi * D;
+  L.lo = w;  w += np * L.nlo * D;
+  L.T0 = w;  w += m * L.nb * D;
+  L.T1e = w; w += L.m_e * RT * D;
+  L.Xe = w;  w += L.m_e * D;
+  L.T1b = w; w += L.m_b * RT;
+  L.Xb = w;  w += L.m_b;
+  L.words = w;
+  L.entries = np * (L.nhi + L.nlo) + m * L.nb + L.m_e * (RT + 1) + L.m_b * (RT + 1);
+  DevBuf buf(L.words * 8, s);
+  if (!in.empty()) upload_async(buf.ptr, in.data(), in.size() * 8, s);
+  k_cov_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
   WHIR_HIP(hipGetLastError());
+  ProfScope prof(kernel_name("k_cov", D, K), s, 8.0 * D * (accumulate ? 2.0 : 1.0) * (double)n);
+  const unsigned g = (unsigned)L.nb;
+  const int ac = accumulate ? 1 : 0;
+  uint64_t* tb = buf.as<uint64_t>();
+  switch (K) {
+    case 16: k_cov<D, 16><<<g, RT, 0, s>>>(acc, n, ac, tb, L); break;
+    case 8: k_cov<D, 8><<<g, RT, 0, s>>>(acc, n, ac, tb, L); break;
+    case 4: k_cov<D, 4><<<g, RT, 0, s>>>(acc, n, ac, tb, L); break;
+    case 2: k_cov<D, 2><<<g, RT, 0, s>>>(acc, n, ac, tb, L); break;
+    default: k_cov<D, 1><<<g, RT, 0, s>>>(acc, n, ac, tb, L); break;
+  }
+  WHIR_HIP(hipGetLastError());
+}
+
+void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_t* eq_points,
+                  const uint64_t* eq_scalars, size_t np, size_t nv, const uint64_t* scalars, const uint64_t* points,
+                  size_t m, hipStream_t s) {
+  switch (D) {
+    case 1: covector_t<1>(acc, n, accumulate, eq_points, eq_scalars, np, nv, scalars, points, m, s); break;
+    case 2: covector_t<2>(acc, n, accumulate, eq_points, eq_scalars, np, nv, scalars, points, m, s); break;
+    case 3: covector_t<3>(acc, n, accumulate, eq_points, eq_scalars, np, nv, scalars, points, m, s); break;
+    default: throw Error("covector: extension degree must be 1, 2 or 3");
+  }
+}
+
+void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t s) {
+  covector_dev(D, acc, (size_t)1 << k, true, point, scalar, 1, k, nullptr, nullptr, 0, s);
 }
 
 void geometric_accumulate_dev(int D, uint64_t* acc, size_t n, const uint64_t* scalars, const uint64_t* points,
                               size_t m, hipStream_t s) {
-  switch (D) {
-    case 1: geom_t<1>(acc, n, scalars, points, m, s); break;
-    case 2: geom_t<2>(acc, n, scalars, points, m, s); break;
-    case 3: geom_t<3>(acc, n, scalars, points, m, s); break;
-    default: throw Error("geometric_accumulate: extension degree must be 1, 2 or 3");
-  }
+  covector_dev(D, acc, n, true, nullptr, nullptr, 0, 0, scalars, points, m, s);
 }
 
 // ------------------------------------------------------------------ univariate evaluation
-// Same lane layout as geometric_accumulate: lane partial = sum_k c_i x^i.
-template <int Ds, int D>
-__global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, Ext<D> xbase_step, const uint64_t* xb,
-                                            const uint64_t* xt, Ext<D> step, uint64_t* parts) {
-  const size_t b = blockIdx.x;
-  const size_t i0 = b * (size_t)RT * GA_K + threadIdx.x;
-  Ext<D> p = gl::emul<D>(ld<D>(xb, b), ld<D>(xt, threadIdx.x));
-  Ext<D> acc[1] = {gl::ext_zero<D>()};
-#pragma unroll
-  for (int k = 0; k < GA_K; k++) {
-    size_t i = i0 + (size_t)k * RT;
-    if (i < n) {
-      if (Ds == 1) {
-        acc[0] = gl::eadd<D>(acc[0], gl::escale<D>(p, c[i]));
-      } else {
-        acc[0] = gl::eadd<D>(acc[0], gl::emul<D>(p, ld<D>(c, i)));
-      }
-    }
-    p = gl::emul<D>(p, step);
+// mixed_univariate_evaluate (src/algebra/mod.rs:105-129) at up to UNI_MP points per
+// pass: f(x_j) = sum_b x_j^(b S) sum_{q < S} c_(b S + q) x_j^q, S = RT UNI_K.  Block
+// b sums its segment against the power table P[j][q] = x_j^q (one base-by-extension
+// or extension product per element and point, no chained step); the final kernel
+// scales block partials by B[j][b] = x_j^(b S) and adds them.
+static constexpr int UNI_K = 16;
+static constexpr int UNI_MP = 4;
+
+struct UniPlan {
+  size_t m, nb, S;
+  size_t in_x, P, B, words, entries;
+};
+
+template <int D>
+__global__ void k_uni_tables(uint64_t* buf, UniPlan L) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= L.entries) return;
+  const uint64_t* xs = buf + L.in_x;
+  if (e < L.m * L.S) {
+    st<D>(buf + L.P, e, gl::epow<D>(ld<D>(xs, e / L.S), e % L.S));
+  } else {
+    const size_t f = e - L.m * L.S;
+    st<D>(buf + L.B, f, gl::epow<D>(ld<D>(xs, f / L.nb), (uint64_t)(f % L.nb) * L.S));
   }
-  (void)xbase_step;
-  block_reduce_store<D, 1>(acc, parts + b * D);
 }
 
 template <int Ds, int D>
-static void uni_t(const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out, hipStream_t s) {
-  if (!n) {
-    std::vector<uint64_t> z(D, 0);
-    upload_async(out, z.data(), D * 8, s);
-    return;
+__global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, const uint64_t* buf, UniPlan L, int m,
+                                            uint64_t* parts) {
+  const size_t b = blockIdx.x;
+  const size_t i0 = b * L.S + threadIdx.x;
+  Ext<D> acc[UNI_MP];
+#pragma unroll
+  for (int j = 0; j < UNI_MP; j++) acc[j] = gl::ext_zero<D>();
+#pragma unroll
+  for (int k = 0; k < UNI_K; k++) {
+    const size_t i = i0 + (size_t)k * RT;
+    if (i >= n) break;
+    const size_t q = (size_t)k * RT + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < UNI_MP; j++) {
+      if (j < m) {
+        const Ext<D> pw = ld<D>(buf + L.P, j * L.S + q);
+        if constexpr (Ds == 1) acc[j] = gl::eadd<D>(acc[j], gl::escale<D>(pw, c[i]));
+        else acc[j] = gl::eadd<D>(acc[j], gl::emul<D>(pw, ld<D>(c, i)));
+      }
+    }
   }
-  size_t span = (size_t)RT * GA_K;
-  size_t nb = (n + span - 1) / span;
-  Ext<D> xx = host_ext<D>(x);
-  Ext<D> xspan = gl::epow<D>(xx, span);
-  std::vector<uint64_t> xb(nb * D), xt(RT * D);
-  Ext<D> cur = gl::ext_base<D>(1);
-  for (size_t b = 0; b < nb; b++) {
-    for (int d = 0; d < D; d++) xb[b * D + d] = cur.c[d];
-    cur = gl::emul<D>(cur, xspan);
-  }
-  Ext<D> pw = gl::ext_base<D>(1);
-  for (int t = 0; t < RT; t++) {
-    for (int d = 0; d < D; d++) xt[t * D + d] = pw.c[d];
-    pw = gl::emul<D>(pw, xx);
-  }
-  DevBuf db(xb.size() * 8, s), dt(xt.size() * 8, s), parts(nb * D * 8, s);
-  upload_async(db.ptr, xb.data(), xb.size() * 8, s);
-  upload_async(dt.ptr, xt.data(), xt.size() * 8, s);
-  {
-    ProfScope prof("univariate_eval", s, 8.0 * Ds * (double)n);
-    k_uni<Ds, D><<<(unsigned)nb, RT, 0, s>>>(c, n, xspan, db.as<uint64_t>(), dt.as<uint64_t>(), pw,
-                                             parts.as<uint64_t>());
+  block_reduce_store<D, UNI_MP>(acc, parts + b * UNI_MP * D);
+}
+
+template <int D>
+__global__ __launch_bounds__(RT) void k_uni_final(const uint64_t* parts, const uint64_t* buf, UniPlan L, int m,
+                                                  uint64_t* out) {
+  Ext<D> v[UNI_MP];
+#pragma unroll
+  for (int j = 0; j < UNI_MP; j++) v[j] = gl::ext_zero<D>();
+  for (size_t b = threadIdx.x; b < L.nb; b += blockDim.x)
+#pragma unroll
+    for (int j = 0; j < UNI_MP; j++)
+      if (j < m) v[j] = gl::eadd<D>(v[j], gl::emul<D>(ld<D>(parts + b * UNI_MP * D, j), ld<D>(buf + L.B, j * L.nb + b)));
+  __shared__ uint64_t res[UNI_MP * D];
+  block_reduce_store<D, UNI_MP>(v, res);
+  __syncthreads();
+  if (threadIdx.x < (unsigned)(m * D)) out[threadIdx.x] = res[threadIdx.x];
+}
+
+template <int Ds, int D>
+static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uint64_t* out, hipStream_t s) {
+  for (size_t j0 = 0; j0 < m; j0 += UNI_MP) {
+    const size_t mj = std::min<size_t>(UNI_MP, m - j0);
+    uint64_t* o = out + j0 * D;
+    if (!n) {
+      fill_zero(o, mj * D * 8, s);
+      continue;
+    }
+    UniPlan L{};
+    L.m = mj;
+    L.S = (size_t)RT * UNI_K;
+    L.nb = (n + L.S - 1) / L.S;
+    L.in_x = 0;
+    L.P = mj * D;
+    L.B = L.P + mj * L.S * D;
+    L.words = L.B + mj * L.nb * D;
+    L.entries = mj * (L.S + L.nb);
+    DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s);
+    upload_async(buf.ptr, xs + j0 * D, mj * D * 8, s);
+    k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
+    WHIR_HIP(hipGetLastError());
+    {
+      ProfScope prof(kernel_name("k_uni", Ds, D), s, 8.0 * Ds * (double)n);
+      k_uni<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, buf.as<uint64_t>(), L, (int)mj, parts.as<uint64_t>());
+      WHIR_HIP(hipGetLastError());
+    }
+    k_uni_final<D><<<1, RT, 0, s>>>(parts.as<uint64_t>(), buf.as<uint64_t>(), L, (int)mj, o);
     WHIR_HIP(hipGetLastError());
   }
-  finish_reduce<D, 1>(parts.as<uint64_t>(), (unsigned)nb, out, s);
+}
+
+void univariate_eval_many_dev(int Ds, int D, const uint64_t* c, size_t n, const uint64_t* xs, size_t m,
+                              uint64_t* out, hipStream_t s) {
+  if (!m) return;
+  if (Ds == 1) {
+    switch (D) {
+      case 1: uni_t<1, 1>(c, n, xs, m, out, s); return;
+      case 2: uni_t<1, 2>(c, n, xs, m, out, s); return;
+      case 3: uni_t<1, 3>(c, n, xs, m, out, s); return;
+    }
+  } else if (Ds == D) {
+    switch (D) {
+      case 2: uni_t<2, 2>(c, n, xs, m, out, s); return;
+      case 3: uni_t<3, 3>(c, n, xs, m, out, s); return;
+    }
+  }
+  throw Error("univariate_evaluate: unsupported degrees");
 }
 
 void univariate_eval_dev(int Ds, int D, const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out,
                          hipStream_t s) {
-  if (Ds == 1) {
-    switch (D) {
-      case 1: uni_t<1, 1>(c, n, x, out, s); return;
-      case 2: uni_t<1, 2>(c, n, x, out, s); return;
-      case 3: uni_t<1, 3>(c, n, x, out, s); return;
-    }
-  } else if (Ds == D) {
-    switch (D) {
-      case 2: uni_t<2, 2>(c, n, x, out, s); return;
-      case 3: uni_t<3, 3>(c, n, x, out, s); return;
-    }
-  }
-  throw Error("univariate_evaluate: unsupported degrees");
+  univariate_eval_many_dev(Ds, D, c, n, x, 1, out, s);
 }
 
 // ------------------------------------------------------------------ lift, axpy, dot, gather
@@ -482,7 +591,7 @@ __global__ void k_lift(const uint64_t* src, size_t n, uint64_t* dst) {
 
 void lift_dev(int D, const uint64_t* src, size_t n, uint64_t* dst, hipStream_t s) {
   if (!n) return;
-  ProfScope prof("lift", s, 8.0 * (1.0 + D) * (double)n);
+  ProfScope prof(kernel_name("k_lift", D), s, 8.0 * (1.0 + D) * (double)n);
   switch (D) {
     case 1: k_lift<1><<<grid_for(n), RT, 0, s>>>(src, n, dst); break;
     case 2: k_lift<2><<<grid_for(n), RT, 0, s>>>(src, n, dst); break;

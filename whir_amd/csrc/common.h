@@ -90,6 +90,9 @@ bool profile_on();
 void profile_set(bool on);
 void profile_reset();
 std::string profile_json();  // {"name": {"launches", "total_ms", "bytes"}}
+// Interned "base<a>" / "base<a, b>": the profiler names a launch after its kernel
+// template as rocprofv3 prints it, so the two reports line up.
+const char* kernel_name(const char* base, int a, int b = -1);
 struct ProfScope {
   ProfScope(const char* name, hipStream_t s, double bytes);
   ~ProfScope();

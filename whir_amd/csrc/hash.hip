@@ -107,7 +107,8 @@ template <int D>
 static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st) {
   size_t W = M * D;
   int RB = (int)std::min<size_t>(LEAF_THREADS, 8192 / (W + 1));
-  ProfScope prof("leaf_hash", st, 8.0 * (double)W * N + 32.0 * N);
+  ProfScope prof(kernel_name(RB >= 16 ? "k_leaves_cols" : "k_leaves_cols_wide", D), st,
+                 8.0 * (double)W * N + 32.0 * N);
   if (RB >= 16) {
     size_t smem = (size_t)RB * (W + 1) * 8;
     k_leaves_cols<D><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(engine, cw, N, M, leaves, RB);
@@ -218,7 +219,7 @@ void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t
     while (lv < maxlev && ((size_t)1 << (lv + 1)) <= w) lv++;
     size_t blocks = w >> lv;
     int threads = std::max(64, std::min(MERKLE_THREADS, 1 << (lv - 1)));
-    ProfScope prof("merkle", st, 32.0 * (double)(2 * w - (w >> lv)));
+    ProfScope prof("k_merkle_levels", st, 32.0 * (double)(2 * w - (w >> lv)));
     k_merkle_levels<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, lv);
     WHIR_HIP(hipGetLastError());
     w >>= lv;

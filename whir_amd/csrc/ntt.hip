@@ -520,14 +520,6 @@ static int floor_log2(int x) {
   return k;
 }
 
-static const char* dname(const char* base, int D) {
-  static std::mutex mu;
-  static std::map<std::pair<std::string, int>, std::string> names;
-  std::lock_guard<std::mutex> g(mu);
-  auto& s = names[{base, D}];
-  if (s.empty()) s = std::string(base) + "<" + std::to_string(D) + ">";
-  return s.c_str();
-}
 
 template <int D, int EPT>
 static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
@@ -545,7 +537,7 @@ static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, i
   size_t smem = ((size_t)tile * D + C2) * 8;
   dim3 grid(xcd_grid(base_blocks * ((a.M + jn - 1) / jn)));
   allow_lds(k_ntt_pass1<D, EPT>, smem);
-  ProfScope prof(dname("ntt_pass1", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
+  ProfScope prof(kernel_name("k_ntt_pass1", D, EPT), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
   k_ntt_pass1<D, EPT><<<grid, NTT_THREADS, smem, st>>>(a, row, post, flat_twiddles(logC2), scratch, logC1, logb, jn);
   WHIR_HIP(hipGetLastError());
 }
@@ -564,7 +556,7 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     size_t smem = ((size_t)G * C * D + C) * 8;
     unsigned blocks = (unsigned)((nseq + G - 1) / G);
     allow_lds(k_ntt_single<D>, smem);
-    ProfScope prof(dname("ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
+    ProfScope prof(kernel_name("k_ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
     k_ntt_single<D><<<blocks, NTT_THREADS, smem, st>>>(a, pre, twS, G);
     WHIR_HIP(hipGetLastError());
     return;
@@ -581,7 +573,7 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     size_t smem = (((size_t)C1 << logb) * D + C1) * 8;
     dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
     allow_lds(k_ntt_pass2<D>, smem);
-    ProfScope prof(dname("ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
+    ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
     k_ntt_pass2<D><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
     WHIR_HIP(hipGetLastError());
   }

@@ -182,13 +182,14 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec,
   w.ood_points.clear();
   w.ood_values.clear();
   for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
-  DevBuf out(D * 8, s);
-  for (size_t i = 0; i < c.out_domain_samples; i++) {
-    univariate_eval_dev(Ds, D, vec, c.vector_size, &w.ood_points[i * D], out.as<uint64_t>(), s);
-    E<D> v;
-    d2h<D>(v, out.as<uint64_t>(), s);
-    t.prover_field(D, v.c, 1);
-    push<D>(w.ood_values, v);
+  const size_t nood = c.out_domain_samples;
+  if (nood) {
+    // All samples in shared passes over the vector, one read-back.
+    DevBuf out(nood * D * 8, s);
+    univariate_eval_many_dev(Ds, D, vec, c.vector_size, w.ood_points.data(), nood, out.as<uint64_t>(), s);
+    w.ood_values.resize(nood * D);
+    download_sync(w.ood_values.data(), out.ptr, nood * D * 8, s);
+    for (size_t i = 0; i < nood; i++) t.prover_field(D, &w.ood_values[i * D], 1);
   }
 }
 
@@ -313,16 +314,17 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
   const size_t nood = w0.ood_points.size() / D;
   std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
   const bool has_constraints = !crlc.empty();
-  fill_zero(cov.ptr, n * D * 8, s);
-  for (size_t k = 0; k < np; k++) eval_eq_dev(D, cov.as<uint64_t>(), points + k * nv * D, nv, crlc[k].c, s);
+  {
+    // cov = sum_k crlc_k eq(point_k, .) + sum_o crlc_(np+o) (1, x_o, x_o^2, ...): one pass.
+    std::vector<uint64_t> erlc, srlc;
+    for (size_t k = 0; k < np; k++) push<D>(erlc, crlc[k]);
+    for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
+    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), np, nv, srlc.data(),
+                 w0.ood_points.data(), nood, s);
+  }
   E<D> the_sum = gl::ext_zero<D>();
   for (size_t k = 0; k < np; k++)
     the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[k], gl::emul<D>(vector_rlc[0], ld<D>(evals + k * D))));
-  if (nood) {
-    std::vector<uint64_t> srlc;
-    for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
-    geometric_accumulate_dev(D, cov.as<uint64_t>(), n, srlc.data(), w0.ood_points.data(), nood, s);
-  }
   for (size_t k = 0; k < nood; k++)
     the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[np + k],
                                                gl::emul<D>(vector_rlc[0], ld<D>(&w0.ood_values[k * D]))));
@@ -367,7 +369,8 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
     if (!srlc.empty()) {
       std::vector<uint64_t> flat;
       for (auto& e : srlc) push<D>(flat, e);
-      geometric_accumulate_dev(D, cov.as<uint64_t>(), n, flat.data(), spoints.data(), srlc.size(), s);
+      covector_dev(D, cov.as<uint64_t>(), n, true, nullptr, nullptr, 0, 0, flat.data(), spoints.data(),
+                   srlc.size(), s);
     }
     for (size_t k = 0; k < srlc.size(); k++) the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], svals[k]));
     fr = sumcheck_prove<D>(t, rc.sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);

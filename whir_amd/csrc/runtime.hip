@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace whir {
@@ -216,6 +217,19 @@ hipEvent_t take_event() {
 }  // namespace
 
 bool profile_on() { return g_prof_on; }
+
+const char* kernel_name(const char* base, int a, int b) {
+  static std::mutex mu;
+  static std::map<std::tuple<std::string, int, int>, std::string> names;
+  std::lock_guard<std::mutex> g(mu);
+  auto& s = names[std::make_tuple(std::string(base), a, b)];
+  if (s.empty()) {
+    s = std::string(base) + "<" + std::to_string(a);
+    if (b >= 0) s += ", " + std::to_string(b);
+    s += ">";
+  }
+  return s.c_str();
+}
 
 ProfScope::ProfScope(const char* n, hipStream_t st, double b) : name(n), s(st), bytes(b) {
   if (!g_prof_on) return;
