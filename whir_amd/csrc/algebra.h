@@ -20,6 +20,16 @@ size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t st)
 size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* dev_out,
                          hipStream_t st);
 
+// The same three with a first operand of Ds words per element (Ds = 1: the
+// base-field vector of the initial sumcheck round, read without lifting it).
+// Folding writes the folded a to a_out (Ds = D: may be a_in itself).
+void sumcheck_poly_mixed_dev(int Ds, int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* dev_out,
+                             hipStream_t st);
+size_t fold_into_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, size_t n, const uint64_t* w,
+                     hipStream_t st);
+size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n,
+                               const uint64_t* w, uint64_t* dev_out, hipStream_t st);
+
 // eval_eq (src/algebra/multilinear.rs:170-190): acc[x] += s * prod_i eq(z_i, x_i), x_0 = MSB.
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t st);
 

@@ -57,7 +57,11 @@ __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out) {
     for (int k = 0; k < NV * D; k++) out[k] = red[k];
 }
 
-// Final reduction of `nparts` partials of NV*D words each.
+// Final reduction of `nparts` block partials of NV*D words each into `out`
+// (device memory, or the stream's pinned mailbox).  A second launch rather than a
+// last-block-done reduction in the first: on CDNA4 the per-block device-scope
+// release that pattern needs writes back the XCD's L2, and the producing kernels
+// have just filled it with their outputs (measured 1.5x slower on MI355X).
 template <int D, int NV>
 __global__ __launch_bounds__(RT) void k_reduce_parts(const uint64_t* parts, size_t nparts, uint64_t* out) {
   Ext<D> v[NV];
@@ -69,13 +73,13 @@ __global__ __launch_bounds__(RT) void k_reduce_parts(const uint64_t* parts, size
   block_reduce_store<D, NV>(v, out);
 }
 
-static unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(blocks_for(n, RT), MAX_BLOCKS); }
-
 template <int D, int NV>
 static void finish_reduce(const uint64_t* parts, unsigned nparts, uint64_t* out, hipStream_t s) {
   k_reduce_parts<D, NV><<<1, RT, 0, s>>>(parts, nparts, out);
   WHIR_HIP(hipGetLastError());
 }
+
+static unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(blocks_for(n, RT), MAX_BLOCKS); }
 
 // Pass a small host constant (D words) to kernels by value.
 template <int D>
@@ -92,36 +96,55 @@ static size_t next_pow2(size_t n) {
 }
 
 // ------------------------------------------------------------------ sumcheck polynomial
-template <int D>
+// Element i of a vector of Ds-word elements, embedded in the degree-D extension
+// (Ds = 1: a base-field vector, as the initial round sees the committed vector).
+template <int Ds, int D>
+__device__ __forceinline__ Ext<D> ldx(const uint64_t* p, size_t i) {
+  if constexpr (Ds == D) return ld<D>(p, i);
+  else return gl::ext_base<D>(p[i]);
+}
+
+// a * b with a embedded from Ds words.
+template <int Ds, int D>
+__device__ __forceinline__ Ext<D> mulx(const uint64_t* a, size_t i, const Ext<D>& b) {
+  if constexpr (Ds == D) return gl::emul<D>(ld<D>(a, i), b);
+  else return gl::escale<D>(b, a[i]);
+}
+
+template <int Ds, int D>
 __global__ __launch_bounds__(RT) void k_sc_poly(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
                                                 size_t half, size_t nhi, uint64_t* parts) {
   Ext<D> acc[2] = {gl::ext_zero<D>(), gl::ext_zero<D>()};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x) {
-    Ext<D> a0 = ld<D>(a, i), b0 = ld<D>(b, i);
-    Ext<D> p0 = gl::emul<D>(a0, b0);
+    const Ext<D> b0 = ld<D>(b, i);
+    const Ext<D> p0 = mulx<Ds, D>(a, i, b0);
     acc[0] = gl::eadd<D>(acc[0], p0);
     if (i < nhi) {
-      Ext<D> da = gl::esub<D>(ld<D>(a, half + i), a0), db = gl::esub<D>(ld<D>(b, half + i), b0);
-      acc[1] = gl::eadd<D>(acc[1], gl::emul<D>(da, db));
+      const Ext<D> db = gl::esub<D>(ld<D>(b, half + i), b0);
+      if constexpr (Ds == D) {
+        acc[1] = gl::eadd<D>(acc[1], gl::emul<D>(gl::esub<D>(ld<D>(a, half + i), ld<D>(a, i)), db));
+      } else {
+        acc[1] = gl::eadd<D>(acc[1], gl::escale<D>(db, gl::sub(a[half + i], a[i])));
+      }
     } else {
       acc[1] = gl::eadd<D>(acc[1], p0);  // implicit zero padding: (0 - a)(0 - b) = a b
     }
   }
-  block_reduce_store<D, 2>(acc, parts + blockIdx.x * 2 * D);
+  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D);
 }
 
-template <int D>
+template <int Ds, int D>
 __global__ void k_sc_single(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out) {
   if (threadIdx.x != 0) return;
-  Ext<D> c0 = n ? gl::emul<D>(ld<D>(a, 0), ld<D>(b, 0)) : gl::ext_zero<D>();
+  Ext<D> c0 = n ? mulx<Ds, D>(a, 0, ld<D>(b, 0)) : gl::ext_zero<D>();
   st<D>(out, 0, c0);
   st<D>(out, 1, gl::ext_zero<D>());
 }
 
-template <int D>
+template <int Ds, int D>
 static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out, hipStream_t s) {
   if (n <= 1) {
-    k_sc_single<D><<<1, 64, 0, s>>>(a, b, n, out);
+    k_sc_single<Ds, D><<<1, 64, 0, s>>>(a, b, n, out);
     WHIR_HIP(hipGetLastError());
     return;
   }
@@ -129,90 +152,128 @@ static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* ou
   unsigned g = grid_for(half);
   DevBuf parts((size_t)g * 2 * D * 8, s);
   {
-    ProfScope prof(kernel_name("k_sc_poly", D), s, 16.0 * D * (double)n);
-    k_sc_poly<D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
+    ProfScope prof(kernel_name("k_sc_poly", Ds, D), s, 8.0 * (Ds + D) * (double)n);
+    k_sc_poly<Ds, D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
     WHIR_HIP(hipGetLastError());
   }
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
 }
 
-void sumcheck_poly_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out, hipStream_t s) {
-  switch (D) {
-    case 1: sc_poly<1>(a, b, n, out, s); break;
-    case 2: sc_poly<2>(a, b, n, out, s); break;
-    case 3: sc_poly<3>(a, b, n, out, s); break;
-    default: throw Error("sumcheck: extension degree must be 1, 2 or 3");
+void sumcheck_poly_mixed_dev(int Ds, int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out,
+                             hipStream_t s) {
+  if (Ds == D) {
+    switch (D) {
+      case 1: sc_poly<1, 1>(a, b, n, out, s); return;
+      case 2: sc_poly<2, 2>(a, b, n, out, s); return;
+      case 3: sc_poly<3, 3>(a, b, n, out, s); return;
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: sc_poly<1, 2>(a, b, n, out, s); return;
+      case 3: sc_poly<1, 3>(a, b, n, out, s); return;
+    }
   }
+  throw Error("sumcheck: extension degree must be 1, 2 or 3");
+}
+
+void sumcheck_poly_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out, hipStream_t s) {
+  sumcheck_poly_mixed_dev(D, D, a, b, n, out, s);
 }
 
 // ------------------------------------------------------------------ fold
-template <int D>
-__global__ __launch_bounds__(RT) void k_fold(uint64_t* v, size_t half, size_t nhi, Ext<D> w, Ext<D> one_minus_w) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x) {
-    Ext<D> lo = ld<D>(v, i);
+// Folded element i of v (pairs (i, half + i), zero padded past n):
+//   v[i] + w (v[half + i] - v[i])  or  v[i] (1 - w) when the partner is padding.
+template <int Ds, int D>
+__device__ __forceinline__ Ext<D> fold_at(const uint64_t* v, size_t i, size_t half, size_t nhi, const Ext<D>& w,
+                                          const Ext<D>& om) {
+  if constexpr (Ds == D) {
+    const Ext<D> lo = ld<D>(v, i);
+    if (i < nhi) return gl::eadd<D>(lo, gl::emul<D>(gl::esub<D>(ld<D>(v, half + i), lo), w));
+    return gl::emul<D>(lo, om);
+  } else {
+    const uint64_t lo = v[i];
     if (i < nhi) {
-      Ext<D> hi = ld<D>(v, half + i);
-      st<D>(v, i, gl::eadd<D>(lo, gl::emul<D>(gl::esub<D>(hi, lo), w)));
-    } else {
-      st<D>(v, i, gl::emul<D>(lo, one_minus_w));
+      Ext<D> r = gl::escale<D>(w, gl::sub(v[half + i], lo));
+      r.c[0] = gl::add(r.c[0], lo);
+      return r;
     }
+    return gl::escale<D>(om, lo);
   }
 }
 
-template <int D>
-static size_t fold_t(uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) {
-  if (n <= 1) return n;
+// out may alias in when Ds == D: lane i reads positions i and half + i and writes i only.
+template <int Ds, int D>
+__global__ __launch_bounds__(RT) void k_fold(const uint64_t* in, uint64_t* out, size_t half, size_t nhi, Ext<D> w,
+                                             Ext<D> one_minus_w) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x)
+    st<D>(out, i, fold_at<Ds, D>(in, i, half, nhi, w, one_minus_w));
+}
+
+template <int Ds, int D>
+static size_t fold_t(const uint64_t* in, uint64_t* out, size_t n, const uint64_t* w, hipStream_t s) {
+  if (n <= 1) {
+    if (n == 1 && Ds != D) lift_dev(D, in, 1, out, s);  // nothing to fold: embed
+    if (n == 1 && Ds == D && in != out) WHIR_HIP(hipMemcpyAsync(out, in, D * 8, hipMemcpyDeviceToDevice, s));
+    return n;
+  }
   size_t half = next_pow2(n) >> 1, nhi = n - half;
   Ext<D> ww = host_ext<D>(w);
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
-  ProfScope prof(kernel_name("k_fold", D), s, 8.0 * D * ((double)n + (double)half));
-  k_fold<D><<<grid_for(half), RT, 0, s>>>(v, half, nhi, ww, om);
+  ProfScope prof(kernel_name("k_fold", Ds, D), s, 8.0 * ((double)Ds * n + (double)D * half));
+  k_fold<Ds, D><<<grid_for(half), RT, 0, s>>>(in, out, half, nhi, ww, om);
   WHIR_HIP(hipGetLastError());
   return half;
 }
 
-size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) {
-  switch (D) {
-    case 1: return fold_t<1>(v, n, w, s);
-    case 2: return fold_t<2>(v, n, w, s);
-    case 3: return fold_t<3>(v, n, w, s);
+size_t fold_into_dev(int Ds, int D, const uint64_t* in, uint64_t* out, size_t n, const uint64_t* w, hipStream_t s) {
+  if (Ds == D) {
+    switch (D) {
+      case 1: return fold_t<1, 1>(in, out, n, w, s);
+      case 2: return fold_t<2, 2>(in, out, n, w, s);
+      case 3: return fold_t<3, 3>(in, out, n, w, s);
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: return fold_t<1, 2>(in, out, n, w, s);
+      case 3: return fold_t<1, 3>(in, out, n, w, s);
+    }
   }
   throw Error("fold: extension degree must be 1, 2 or 3");
 }
 
-// ------------------------------------------------------------------ fused fold + polynomial
-// Folded length h = next_pow2(n)/2 (a power of two >= 2); new pairs (i, h/2 + i).
-template <int D>
-__device__ __forceinline__ Ext<D> fold_at(const uint64_t* v, size_t i, size_t h, size_t nhi, const Ext<D>& w,
-                                          const Ext<D>& om) {
-  Ext<D> lo = ld<D>(v, i);
-  if (i < nhi) return gl::eadd<D>(lo, gl::emul<D>(gl::esub<D>(ld<D>(v, h + i), lo), w));
-  return gl::emul<D>(lo, om);
+size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) {
+  return fold_into_dev(D, D, v, v, n, w, s);
 }
 
-template <int D>
-__global__ __launch_bounds__(RT) void k_fold_poly(uint64_t* a, uint64_t* b, size_t h, size_t nhi, Ext<D> w,
-                                                  Ext<D> om, uint64_t* parts) {
+// ------------------------------------------------------------------ fused fold + polynomial
+// Fold a (Ds words, into a_out) and b (in place) by w, then the round polynomial of
+// the folded pair.  Folded length h = next_pow2(n)/2 (a power of two >= 2); the new
+// pairs are (i, h/2 + i).  Lane i writes positions i and h/2 + i only and reads
+// i, h/2 + i, h + i, h + h/2 + i, so a_out may alias a_in when Ds == D.
+template <int Ds, int D>
+__global__ __launch_bounds__(RT) void k_fold_poly(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t h,
+                                                  size_t nhi, Ext<D> w, Ext<D> om, uint64_t* parts) {
   Ext<D> acc[2] = {gl::ext_zero<D>(), gl::ext_zero<D>()};
   const size_t q = h >> 1;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
-    Ext<D> a0 = fold_at<D>(a, i, h, nhi, w, om), a1 = fold_at<D>(a, q + i, h, nhi, w, om);
-    Ext<D> b0 = fold_at<D>(b, i, h, nhi, w, om), b1 = fold_at<D>(b, q + i, h, nhi, w, om);
-    st<D>(a, i, a0); st<D>(a, q + i, a1);
+    Ext<D> a0 = fold_at<Ds, D>(a_in, i, h, nhi, w, om), a1 = fold_at<Ds, D>(a_in, q + i, h, nhi, w, om);
+    Ext<D> b0 = fold_at<D, D>(b, i, h, nhi, w, om), b1 = fold_at<D, D>(b, q + i, h, nhi, w, om);
+    st<D>(a_out, i, a0); st<D>(a_out, q + i, a1);
     st<D>(b, i, b0); st<D>(b, q + i, b1);
     acc[0] = gl::eadd<D>(acc[0], gl::emul<D>(a0, b0));
     acc[1] = gl::eadd<D>(acc[1], gl::emul<D>(gl::esub<D>(a1, a0), gl::esub<D>(b1, b0)));
   }
-  block_reduce_store<D, 2>(acc, parts + blockIdx.x * 2 * D);
+  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D);
 }
 
-template <int D>
-static size_t fold_poly_t(uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* out, hipStream_t s) {
+template <int Ds, int D>
+static size_t fold_poly_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n, const uint64_t* w,
+                          uint64_t* out, hipStream_t s) {
   if (n <= 2) {
     // folded length <= 1: fold then the one-element polynomial
-    size_t m = fold_t<D>(a, n, w, s);
-    fold_t<D>(b, n, w, s);
-    sc_poly<D>(a, b, m, out, s);
+    size_t m = fold_t<Ds, D>(a_in, a_out, n, w, s);
+    fold_t<D, D>(b, b, n, w, s);
+    sc_poly<D, D>(a_out, b, m, out, s);
     return m;
   }
   size_t h = next_pow2(n) >> 1, nhi = n - h;
@@ -221,22 +282,34 @@ static size_t fold_poly_t(uint64_t* a, uint64_t* b, size_t n, const uint64_t* w,
   unsigned g = grid_for(h >> 1);
   DevBuf parts((size_t)g * 2 * D * 8, s);
   {
-    ProfScope prof(kernel_name("k_fold_poly", D), s, 8.0 * D * (2.0 * n + 2.0 * h));
-    k_fold_poly<D><<<g, RT, 0, s>>>(a, b, h, nhi, ww, om, parts.as<uint64_t>());
+    ProfScope prof(kernel_name("k_fold_poly", Ds, D), s, 8.0 * ((Ds + D) * (double)n + 2.0 * D * h));
+    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, parts.as<uint64_t>());
     WHIR_HIP(hipGetLastError());
   }
   finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
   return h;
 }
 
-size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* out,
-                         hipStream_t s) {
-  switch (D) {
-    case 1: return fold_poly_t<1>(a, b, n, w, out, s);
-    case 2: return fold_poly_t<2>(a, b, n, w, out, s);
-    case 3: return fold_poly_t<3>(a, b, n, w, out, s);
+size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n,
+                               const uint64_t* w, uint64_t* out, hipStream_t s) {
+  if (Ds == D) {
+    switch (D) {
+      case 1: return fold_poly_t<1, 1>(a_in, a_out, b, n, w, out, s);
+      case 2: return fold_poly_t<2, 2>(a_in, a_out, b, n, w, out, s);
+      case 3: return fold_poly_t<3, 3>(a_in, a_out, b, n, w, out, s);
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: return fold_poly_t<1, 2>(a_in, a_out, b, n, w, out, s);
+      case 3: return fold_poly_t<1, 3>(a_in, a_out, b, n, w, out, s);
+    }
   }
   throw Error("fold: extension degree must be 1, 2 or 3");
+}
+
+size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* out,
+                         hipStream_t s) {
+  return fold_and_poly_mixed_dev(D, D, a, a, b, n, w, out, s);
 }
 
 // ------------------------------------------------------------------ covector terms
@@ -508,23 +581,10 @@ __global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, const u
       }
     }
   }
-  block_reduce_store<D, UNI_MP>(acc, parts + b * UNI_MP * D);
-}
-
-template <int D>
-__global__ __launch_bounds__(RT) void k_uni_final(const uint64_t* parts, const uint64_t* buf, UniPlan L, int m,
-                                                  uint64_t* out) {
-  Ext<D> v[UNI_MP];
 #pragma unroll
-  for (int j = 0; j < UNI_MP; j++) v[j] = gl::ext_zero<D>();
-  for (size_t b = threadIdx.x; b < L.nb; b += blockDim.x)
-#pragma unroll
-    for (int j = 0; j < UNI_MP; j++)
-      if (j < m) v[j] = gl::eadd<D>(v[j], gl::emul<D>(ld<D>(parts + b * UNI_MP * D, j), ld<D>(buf + L.B, j * L.nb + b)));
-  __shared__ uint64_t res[UNI_MP * D];
-  block_reduce_store<D, UNI_MP>(v, res);
-  __syncthreads();
-  if (threadIdx.x < (unsigned)(m * D)) out[threadIdx.x] = res[threadIdx.x];
+  for (int j = 0; j < UNI_MP; j++)
+    if (j < m) acc[j] = gl::emul<D>(acc[j], ld<D>(buf + L.B, j * L.nb + b));  // segment offset x_j^(b S)
+  block_reduce_store<D, UNI_MP>(acc, parts + (size_t)blockIdx.x * UNI_MP * D);
 }
 
 template <int Ds, int D>
@@ -545,17 +605,19 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     L.B = L.P + mj * L.S * D;
     L.words = L.B + mj * L.nb * D;
     L.entries = mj * (L.S + L.nb);
-    DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s);
+    DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s), res(UNI_MP * D * 8, s);
     upload_async(buf.ptr, xs + j0 * D, mj * D * 8, s);
     k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
     WHIR_HIP(hipGetLastError());
+    // The reduction writes all UNI_MP slots; only the first mj go to the caller.
+    uint64_t* dst = mj == UNI_MP ? o : res.as<uint64_t>();
     {
       ProfScope prof(kernel_name("k_uni", Ds, D), s, 8.0 * Ds * (double)n);
       k_uni<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, buf.as<uint64_t>(), L, (int)mj, parts.as<uint64_t>());
       WHIR_HIP(hipGetLastError());
     }
-    k_uni_final<D><<<1, RT, 0, s>>>(parts.as<uint64_t>(), buf.as<uint64_t>(), L, (int)mj, o);
-    WHIR_HIP(hipGetLastError());
+    finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s);
+    if (dst != o) WHIR_HIP(hipMemcpyAsync(o, dst, mj * D * 8, hipMemcpyDeviceToDevice, s));
   }
 }
 
@@ -623,7 +685,7 @@ __global__ __launch_bounds__(RT) void k_dot(const uint64_t* a, const uint64_t* b
   Ext<D> acc[1] = {gl::ext_zero<D>()};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     acc[0] = gl::eadd<D>(acc[0], gl::emul<D>(ld<D>(a, i), ld<D>(b, i)));
-  block_reduce_store<D, 1>(acc, parts + blockIdx.x * D);
+  block_reduce_store<D, 1>(acc, parts + (size_t)blockIdx.x * D);
 }
 
 template <int D>

@@ -41,6 +41,12 @@ void upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Copy device->host through a pinned per-thread buffer and wait for it.
 void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
 
+// Pinned host words (RESULT_MAILBOX_WORDS per stream) that kernels write small
+// results into directly, saving a device-to-host copy per result; read them after
+// synchronizing the stream.
+constexpr size_t RESULT_MAILBOX_WORDS = 4096;
+uint64_t* result_mailbox(hipStream_t s);
+
 // Device buffers recycled per stream: a buffer released on stream s goes to s's
 // free list and is handed out again only to allocations on s, so stream order
 // alone keeps its last readers ahead of its next writers.  Sizes repeat from

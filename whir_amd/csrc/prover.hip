@@ -245,20 +245,30 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
   }
 }
 
-// sumcheck::Config::prove (no masks), staggered fold + round polynomial.
+// sumcheck::Config::prove (no masks), staggered fold + round polynomial.  The
+// first operand starts at a_src with a_Ds words per element (1 for the initial
+// round over the committed base-field vector, which is folded into `a` without
+// being lifted first); from the first fold on it lives in `a`.  Each round's
+// (c0, c2) lands in the stream's pinned mailbox from the reducing launch itself.
 template <int D>
-std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, uint64_t* a, uint64_t* b,
-                                 size_t& n, E<D>& sum, hipStream_t s) {
+std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src, int a_Ds,
+                                 uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s) {
   WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
   std::vector<E<D>> chals;
-  DevBuf dout(2 * D * 8, s);
+  uint64_t* mbox = result_mailbox(s);
   bool have_prev = false;
   E<D> prev;
   for (size_t round = 0; round < sc.num_rounds; round++) {
-    if (have_prev) n = fold_and_poly_dev(D, a, b, n, prev.c, dout.as<uint64_t>(), s);
-    else sumcheck_poly_dev(D, a, b, n, dout.as<uint64_t>(), s);
+    if (have_prev) {
+      n = fold_and_poly_mixed_dev(a_Ds, D, a_src, a, b, n, prev.c, mbox, s);
+      a_src = a;
+      a_Ds = D;
+    } else {
+      sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, n, mbox, s);
+    }
+    WHIR_HIP(hipStreamSynchronize(s));
     uint64_t h[2 * D];
-    download_sync(h, dout.ptr, 2 * D * 8, s);
+    memcpy(h, mbox, sizeof h);
     E<D> c0 = ld<D>(h), c2 = ld<D>(h + D);
     E<D> c1 = gl::esub<D>(gl::esub<D>(gl::esub<D>(sum, c0), c0), c2);
     t.prover_field(D, c0.c, 1);
@@ -271,9 +281,11 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     have_prev = true;
   }
   if (have_prev) {
-    size_t m = fold_dev(D, a, n, prev.c, s);
+    size_t m = fold_into_dev(a_Ds, D, a_src, a, n, prev.c, s);
     fold_dev(D, b, n, prev.c, s);
     n = m;
+  } else if (a_src != a) {
+    lift_dev(D, a_src, n, a, s);
   }
   return chals;
 }
@@ -310,7 +322,6 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
   // Random linear combination of the vectors (one vector: [1], no transcript).
   std::vector<E<D>> vector_rlc = geometric_challenge<D>(t, 1);
   DevBuf vec(n * D * 8, s), cov(n * D * 8, s);
-  lift_dev(D, dvector, n, vec.as<uint64_t>(), s);
   const size_t nood = w0.ood_points.size() / D;
   std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
   const bool has_constraints = !crlc.empty();
@@ -330,11 +341,19 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
                                                gl::emul<D>(vector_rlc[0], ld<D>(&w0.ood_values[k * D]))));
   std::vector<E<D>> fr;
   if (has_constraints) {
-    fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+    fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, dvector, 1, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
+                           the_sum, s);
   } else {
     for (size_t i = 0; i < cfg.initial_sumcheck.num_rounds; i++) fr.push_back(verifier_field<D>(t));
     pow_prove(t, eng, cfg.initial_skip_pow, s);
-    for (const auto& f : fr) n = fold_dev(D, vec.as<uint64_t>(), n, f.c, s);
+    const uint64_t* src = dvector;
+    int src_Ds = 1;
+    for (const auto& f : fr) {
+      n = fold_into_dev(src_Ds, D, src, vec.as<uint64_t>(), n, f.c, s);
+      src = vec.as<uint64_t>();
+      src_Ds = D;
+    }
+    if (fr.empty()) lift_dev(D, dvector, n, vec.as<uint64_t>(), s);
     fill_zero(cov.ptr, n * D * 8, s);  // the covector is all zeros
   }
   std::vector<E<D>> eval_point(fr);
@@ -373,7 +392,8 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
                    srlc.size(), s);
     }
     for (size_t k = 0; k < srlc.size(); k++) the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], svals[k]));
-    fr = sumcheck_prove<D>(t, rc.sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+    fr = sumcheck_prove<D>(t, rc.sumcheck, eng, vec.as<uint64_t>(), D, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
+                           the_sum, s);
     eval_point.insert(eval_point.end(), fr.begin(), fr.end());
     held = std::move(nw);
     prev_cfg = &rc.irs;
@@ -389,7 +409,8 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
     std::vector<uint64_t> pts, rows;
     irs_open<D>(t, *prev_cfg, *prev_w, pts, rows, s);
   }
-  fr = sumcheck_prove<D>(t, cfg.final_sumcheck, eng, vec.as<uint64_t>(), cov.as<uint64_t>(), n, the_sum, s);
+  fr = sumcheck_prove<D>(t, cfg.final_sumcheck, eng, vec.as<uint64_t>(), D, vec.as<uint64_t>(), cov.as<uint64_t>(),
+                         n, the_sum, s);
   eval_point.insert(eval_point.end(), fr.begin(), fr.end());
   if (eval_point_out)
     for (size_t i = 0; i < eval_point.size(); i++)

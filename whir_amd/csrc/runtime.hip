@@ -22,12 +22,7 @@ namespace {
 struct ThreadStream {
   hipStream_t s = nullptr;
   int device = -1;
-  ~ThreadStream() {
-    if (s) {
-      pool_trim(s);
-      (void)hipStreamDestroy(s);
-    }
-  }
+  ~ThreadStream();
 };
 thread_local ThreadStream t_stream;
 }  // namespace
@@ -109,6 +104,17 @@ void pool_trim(hipStream_t s) {
   }
   for (void* p : drop) (void)hipFreeAsync(p, s);
   if (!drop.empty()) (void)hipStreamSynchronize(s);
+}
+
+static void release_stream_slots(hipStream_t s);
+
+ThreadStream::~ThreadStream() {
+  if (s) {
+    pool_trim(s);
+    (void)hipStreamSynchronize(s);
+    release_stream_slots(s);
+    (void)hipStreamDestroy(s);
+  }
 }
 
 hipStream_t thread_stream() {
@@ -289,6 +295,29 @@ std::string profile_json() {
     first = false;
   }
   return out + "}";
+}
+
+namespace {
+struct StreamSlots {
+  uint64_t* mailbox = nullptr;
+};
+std::mutex g_slots_mu;
+std::map<hipStream_t, StreamSlots> g_slots;
+}  // namespace
+
+uint64_t* result_mailbox(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_slots_mu);
+  StreamSlots& sl = g_slots[s];
+  if (!sl.mailbox) WHIR_HIP(hipHostMalloc((void**)&sl.mailbox, RESULT_MAILBOX_WORDS * 8, hipHostMallocDefault));
+  return sl.mailbox;
+}
+
+static void release_stream_slots(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_slots_mu);
+  auto it = g_slots.find(s);
+  if (it == g_slots.end()) return;
+  if (it->second.mailbox) (void)hipHostFree(it->second.mailbox);
+  g_slots.erase(it);
 }
 
 void fill_zero(void* dst, size_t bytes, hipStream_t s) {
