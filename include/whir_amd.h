@@ -138,6 +138,13 @@ int whir_matrix_commit(int engine, int degree, const uint64_t *matrix, size_t ro
 int whir_hash_rows_cols_dev(int engine, int degree, const uint64_t *cw, size_t rows, size_t cols,
                             uint8_t *leaves);
 
+/* matrix_commit::Config::commit (src/protocols/matrix_commit.rs:211-232) of a device
+ * codeword in column-major layout: leaf hashes and every Merkle layer into the device
+ * array nodes (whir_merkle_num_nodes(rows) hashes), as irs_commit.rs:292-350 commits
+ * the interleaved codeword. */
+int whir_matrix_commit_cols_dev(int engine, int degree, const uint64_t *cw, size_t rows, size_t cols,
+                                uint8_t *nodes);
+
 /* merkle_tree::Config::open (merkle_tree.rs:145-186): writes the sibling hints
  * (32 bytes each) in transcript order; *num_hints receives their count.  hints
  * must hold count * layers_for_size(num_leaves) hashes. */

@@ -133,6 +133,21 @@ def test_matrix_commit_and_column_leaves(D, engine):
     assert dl.to_array(np.uint8).tobytes() == exp_leaves
 
 
+@pytest.mark.parametrize("D,rows,cols", [(1, 4, 8), (1, 1 << 12, 16), (2, 1 << 11, 16), (3, 1 << 10, 8),
+                                         (2, 8, 4), (1, 1000, 16), (2, 1 << 12, 12), (1, 2, 8), (3, 64, 16)])
+@pytest.mark.parametrize("engine", [W.HASH_BLAKE3, W.HASH_SHA3])
+def test_matrix_commit_cols_dev(D, rows, cols, engine):
+    """Leaves + tree of a column-major codeword (fused BLAKE3 leaf/two-level kernel for
+    power-of-two rows and 64-byte-multiple rows; the staged path otherwise)."""
+    m = rnd_np(D * 1000 + rows + cols + engine, rows * cols * D)
+    exp = O.merkle_commit(engine, O.hash_rows(engine, D, m, rows, cols))
+    cm = np.ascontiguousarray(m.reshape(rows, cols, D).transpose(1, 0, 2)).reshape(-1)
+    dcw = W.DeviceBuffer.from_array(cm)
+    dn = W.DeviceBuffer(len(exp))
+    W.matrix_commit_cols_dev(engine, D, dcw, rows, cols, dn)
+    assert dn.to_array(np.uint8).tobytes() == exp
+
+
 # ------------------------------------------------------------------ sumcheck and covectors
 @pytest.mark.parametrize("D", [1, 2, 3])
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 8, 100, 1 << 12, 5000])

@@ -78,6 +78,7 @@ SIGNATURES = {
     "whir_merkle_commit_dev": [ci, vp, sz],
     "whir_matrix_commit": [ci, ci, u64p, sz, sz, u8p],
     "whir_hash_rows_cols_dev": [ci, ci, vp, sz, sz, vp],
+    "whir_matrix_commit_cols_dev": [ci, ci, vp, sz, sz, vp],
     "whir_merkle_open": [u8p, sz, szp, sz, u8p, szp],
     "whir_sumcheck_polynomial": [ci, u64p, u64p, sz, u64p],
     "whir_sumcheck_polynomial_dev": [ci, vp, vp, sz, u64p],
@@ -462,6 +463,13 @@ def rs_encode_dev(degree: int, d_msgs: DeviceBuffer, num_messages: int, message_
 
 def hash_rows_cols_dev(engine: int, degree: int, d_cw: DeviceBuffer, rows: int, cols: int, d_leaves: DeviceBuffer):
     _call("whir_hash_rows_cols_dev", engine, degree, d_cw.ptr, rows, cols, d_leaves.ptr)
+
+
+def matrix_commit_cols_dev(engine: int, degree: int, d_cw: DeviceBuffer, rows: int, cols: int,
+                           d_nodes: DeviceBuffer):
+    """Leaves and every Merkle layer of a column-major device codeword (the commit
+    irs_commit.rs:292-350 performs) into d_nodes (merkle_num_nodes(rows) hashes)."""
+    _call("whir_matrix_commit_cols_dev", engine, degree, d_cw.ptr, rows, cols, d_nodes.ptr)
 
 
 def merkle_commit_dev(engine: int, d_nodes: DeviceBuffer, num_leaves: int):

@@ -321,6 +321,13 @@ int whir_hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t rows, 
   API_END
 }
 
+int whir_matrix_commit_cols_dev(int engine, int D, const uint64_t* cw, size_t rows, size_t cols, uint8_t* nodes) {
+  API_BEGIN
+  check_degree(D);
+  commit_cols_dev(engine, D, cw, rows, cols, nodes, thread_stream());
+  API_END
+}
+
 int whir_merkle_open(const uint8_t* nodes, size_t num_leaves, const size_t* indices, size_t count, uint8_t* hints,
                      size_t* num_hints) {
   API_BEGIN

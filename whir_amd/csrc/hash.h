@@ -19,6 +19,10 @@ void hash_many_dev(int engine, size_t size, const uint8_t* in, size_t count, uin
 void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves,
                         hipStream_t st);
 
+// Leaf hashes and the whole Merkle tree of a column-major codeword into nodes
+// (merkle_num_nodes(N) hashes, laid out as merkle_build_dev leaves them).
+void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st);
+
 // Leaf hashes of a row-major matrix m[r*M + j].
 void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, uint8_t* leaves,
                    hipStream_t st);

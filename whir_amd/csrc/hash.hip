@@ -157,53 +157,130 @@ void gather_hashes_dev(const uint8_t* nodes, const uint64_t* idx, size_t k, uint
 }
 
 static constexpr int MERKLE_THREADS = 256;
+static constexpr int MERKLE_SPAN_LOG = 11;  // input nodes per block: 256 lanes x 8
 
-// Input level of width `w` at nodes + in_off*32; reduces `levels` levels.  Block b
-// takes input nodes [b*2^levels, (b+1)*2^levels).  Level of width x starts at 2W - 2x.
-__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_levels(int engine, uint8_t* __restrict__ nodes, size_t W,
-                                                                  size_t w, int levels) {
-  __shared__ uint32_t cur[MERKLE_THREADS * 2][8];
-  const size_t span = (size_t)1 << levels;
-  const size_t base = (size_t)blockIdx.x * span;  // first input node of this block (within level)
-  const uint8_t* in = nodes + 32 * (2 * W - 2 * w);
-  // Level 1: pairs straight from global memory.
-  size_t width = w;
-  size_t npairs = span >> 1;
-  for (size_t k = threadIdx.x; k < npairs; k += blockDim.x) {
-    hd::MemReader rd{in + 32 * (base + 2 * k), 64};
-    uint32_t h[8];
-    hd::hash_msg(engine, rd, 64, h);
-    size_t out_idx = (2 * W - 2 * (width >> 1)) + (base >> 1) + k;
-    store_digest(nodes + 32 * out_idx, h);
-#pragma unroll
-    for (int i = 0; i < 8; i++) cur[k][i] = h[i];
+// 64-byte node pair (two 8-word digests) as a hash input.
+struct PairReader {
+  const uint32_t* l;
+  const uint32_t* r;
+  __device__ __forceinline__ uint32_t w32(size_t off) const {
+    const size_t i = off >> 2;
+    return i < 8 ? l[i] : (i < 16 ? r[i - 8] : 0u);
   }
-  width >>= 1;
-  size_t lbase = base >> 1;
-  for (int l = 1; l < levels; l++) {
-    __syncthreads();
-    npairs >>= 1;
-    uint32_t h[8];
-    bool act = threadIdx.x < npairs;
-    if (act) {
-      uint64_t pair[8];
+  __device__ __forceinline__ uint64_t w64(size_t off) const {
+    return (uint64_t)w32(off) | ((uint64_t)w32(off + 4) << 32);
+  }
+};
+
+// BLAKE3 of a 64-byte message: one block, flags CHUNK_START | CHUNK_END | ROOT.
+__device__ __forceinline__ void b3_pair(const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
+  uint32_t m[16];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        pair[i] = (uint64_t)cur[2 * threadIdx.x][2 * i] | ((uint64_t)cur[2 * threadIdx.x][2 * i + 1] << 32);
-        pair[4 + i] = (uint64_t)cur[2 * threadIdx.x + 1][2 * i] | ((uint64_t)cur[2 * threadIdx.x + 1][2 * i + 1] << 32);
-      }
-      hd::WordReader rd{pair, 64};
-      hd::hash_msg(engine, rd, 64, h);
-      size_t out_idx = (2 * W - 2 * (width >> 1)) + (lbase >> 1) + threadIdx.x;
-      store_digest(nodes + 32 * out_idx, h);
-    }
-    __syncthreads();
-    if (act) {
+  for (int i = 0; i < 8; i++) {
+    m[i] = l[i];
+    m[8 + i] = r[i];
+    out[i] = hd::B3_IV[i];
+  }
+  hd::b3_compress(out, m, 64, 0, hd::B3_CHUNK_START | hd::B3_CHUNK_END | hd::B3_ROOT);
+}
+
+__device__ __forceinline__ void hash_pair(int engine, const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
+  if (engine == ENG_BLAKE3) {
+    b3_pair(l, r, out);
+  } else {
+    PairReader rd{l, r};
+    hd::hash_msg(engine, rd, 64, out);
+  }
+}
+
+// Reduce G levels of 2^G consecutive nodes held in h (node index `first` within a
+// level of width `width`), storing every node made at its place (the level of width
+// x starts at node 2W - 2x); the subtree root ends in h[0].
+template <int G>
+__device__ __forceinline__ void reduce_levels(int engine, uint32_t (&h)[8][8], uint8_t* nodes, size_t W,
+                                              size_t width, size_t first) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) cur[threadIdx.x][i] = h[i];
-    }
+  for (int lv = 0; lv < G; lv++) {
     width >>= 1;
-    lbase >>= 1;
+    first >>= 1;
+    const size_t base = 2 * W - 2 * width + first;
+#pragma unroll
+    for (int k = 0; k < (1 << (G - 1 - lv)); k++) {
+      uint32_t o[8];
+      hash_pair(engine, h[2 * k], h[2 * k + 1], o);
+      store_digest(nodes + 32 * (base + k), o);
+#pragma unroll
+      for (int i = 0; i < 8; i++) h[k][i] = o[i];
+    }
+  }
+}
+
+// merkle_tree::Config::commit (src/protocols/merkle_tree.rs:82-127) from a level of
+// width w: block b reduces input nodes [b 2^L, (b+1) 2^L) through L levels.  The
+// first phase has lane i reduce 2^G1 consecutive nodes (G1 = 3: seven hashes per
+// lane, every lane busy) to one in registers; after it, one level per phase (lane i
+// hashes the pair 2i, 2i + 1 from LDS), which keeps the chain of dependent hashes
+// short once few nodes are left.
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8_t* __restrict__ nodes, size_t W,
+                                                               size_t w, int L, int G1) {
+  __shared__ uint32_t buf[2][MERKLE_THREADS][8];
+  size_t width = w;                                 // width of the current level
+  size_t first = (size_t)blockIdx.x << L;           // block's first node within the current level
+  size_t cnt = (size_t)1 << L;                      // block's nodes in the current level
+  int left = L, cur = 0;
+  bool from_global = true;
+  while (left > 0) {
+    const int G = from_global ? (left < G1 ? left : G1) : 1;
+    const size_t lanes = cnt >> G;
+    if (threadIdx.x < lanes) {
+      uint32_t h[8][8];
+      const size_t f = first + ((size_t)threadIdx.x << G);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (k < (1 << G)) {
+          if (from_global) {
+            const uint4* src = (const uint4*)(nodes + 32 * (2 * W - 2 * width + f + k));
+            const uint4 a = src[0], b = src[1];
+            h[k][0] = a.x; h[k][1] = a.y; h[k][2] = a.z; h[k][3] = a.w;
+            h[k][4] = b.x; h[k][5] = b.y; h[k][6] = b.z; h[k][7] = b.w;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) h[k][i] = buf[cur][(threadIdx.x << G) + k][i];
+          }
+        }
+      }
+      if (G == 3) reduce_levels<3>(engine, h, nodes, W, width, f);
+      else if (G == 2) reduce_levels<2>(engine, h, nodes, W, width, f);
+      else reduce_levels<1>(engine, h, nodes, W, width, f);
+#pragma unroll
+      for (int i = 0; i < 8; i++) buf[cur ^ 1][threadIdx.x][i] = h[0][i];
+    }
+    __syncthreads();
+    cur ^= 1;
+    from_global = false;
+    width >>= G;
+    first >>= G;
+    cnt = lanes;
+    left -= G;
+  }
+}
+
+// Merkle levels above a level of width w (a power of two) already in place.
+static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st) {
+  while (w > 1) {
+    // Wide levels: eight nodes per lane first (2^11 per block); otherwise pairs
+    // (2^9 per block), which has the shortest chain of dependent hashes.
+    const int G1 = w >= ((size_t)MERKLE_THREADS << MERKLE_SPAN_LOG) ? 3 : 1;
+    const int maxL = G1 == 3 ? MERKLE_SPAN_LOG : MERKLE_SPAN_LOG - 2;
+    int L = 0;
+    while (L < maxL && ((size_t)2 << L) <= w) L++;
+    const size_t blocks = w >> L;
+    const unsigned threads =
+        (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
+    ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)));
+    k_merkle_sub<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, L, G1);
+    WHIR_HIP(hipGetLastError());
+    w >>= L;
   }
 }
 
@@ -212,18 +289,105 @@ void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t
   size_t layers = merkle_layers(num_leaves);
   size_t W = (size_t)1 << layers;
   if (W > num_leaves) fill_zero(nodes + 32 * num_leaves, 32 * (W - num_leaves), st);
-  size_t w = W;
-  const int maxlev = 9;  // 512 input nodes per block -> 256 lanes on the first level
-  while (w > 1) {
-    int lv = 0;
-    while (lv < maxlev && ((size_t)1 << (lv + 1)) <= w) lv++;
-    size_t blocks = w >> lv;
-    int threads = std::max(64, std::min(MERKLE_THREADS, 1 << (lv - 1)));
-    ProfScope prof("k_merkle_levels", st, 32.0 * (double)(2 * w - (w >> lv)));
-    k_merkle_levels<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, lv);
-    WHIR_HIP(hipGetLastError());
-    w >>= lv;
+  merkle_from(engine, nodes, W, W, st);
+}
+
+// BLAKE3 leaves of a column-major codeword whose rows are single-chunk messages of
+// a whole number of 64-byte blocks (W = M D words, W % 8 == 0, W <= 128), fused
+// with the two Merkle levels above them: lane g hashes rows 4g..4g+3, their two
+// parents and the grandparent, all in registers, reading each row's words straight
+// from HBM (for a fixed word, consecutive lanes read consecutive rows).
+static constexpr int TREE_THREADS = 256;
+
+template <int D>
+__device__ __forceinline__ void row_block(const uint64_t* __restrict__ cw, size_t N, size_t r, size_t b,
+                                          uint32_t m[16]) {
+  if constexpr (D == 2) {
+    const uint4* c4 = (const uint4*)cw;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = c4[(4 * b + q) * N + r];  // element (column 4b + q, row r)
+      m[4 * q] = v.x; m[4 * q + 1] = v.y; m[4 * q + 2] = v.z; m[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const size_t w = 8 * b + q, j = w / D, d = w % D;
+      const uint64_t v = cw[(j * N + r) * D + d];
+      m[2 * q] = (uint32_t)v;
+      m[2 * q + 1] = (uint32_t)(v >> 32);
+    }
   }
+}
+
+template <int D>
+__global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3(const uint64_t* __restrict__ cw, size_t N, size_t M,
+                                                                 uint8_t* __restrict__ nodes) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (4 * g >= N) return;
+  const size_t nb = M * D / 8;
+  uint32_t left[8], par[8];
+#pragma unroll 1
+  for (int k = 0; k < 4; k++) {
+    const size_t r = 4 * g + k;
+    uint32_t cv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) cv[i] = hd::B3_IV[i];
+#pragma unroll 1
+    for (size_t b = 0; b < nb; b++) {
+      uint32_t m[16];
+      row_block<D>(cw, N, r, b, m);
+      const uint32_t flags =
+          (b == 0 ? hd::B3_CHUNK_START : 0u) | (b + 1 == nb ? (hd::B3_CHUNK_END | hd::B3_ROOT) : 0u);
+      hd::b3_compress(cv, m, 64, 0, flags);
+    }
+    store_digest(nodes + 32 * r, cv);
+    if ((k & 1) == 0) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) left[i] = cv[i];
+    } else {
+      uint32_t p[8];
+      b3_pair(left, cv, p);
+      store_digest(nodes + 32 * (N + (r >> 1)), p);
+      if (k == 1) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) par[i] = p[i];
+      } else {
+        uint32_t gp[8];
+        b3_pair(par, p, gp);
+        store_digest(nodes + 32 * (N + N / 2 + g), gp);
+      }
+    }
+  }
+}
+
+static bool fused_tree_ok(int engine, size_t N, size_t W) {
+  return engine == ENG_BLAKE3 && N >= 4 && (N & (N - 1)) == 0 && W % 8 == 0 && W > 0 && W <= 128;
+}
+
+template <int D>
+static void commit_cols_t(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st) {
+  const size_t W = M * D;
+  ProfScope prof(kernel_name("k_leaves_tree_b3", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
+  k_leaves_tree_b3<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+  WHIR_HIP(hipGetLastError());
+}
+
+void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D), "leaf hash: engine does not support the row size");
+  WHIR_REQUIRE(N > 0, "commit: empty codeword");
+  if (!fused_tree_ok(engine, N, M * D)) {
+    hash_rows_cols_dev(engine, D, cw, N, M, nodes, st);
+    merkle_build_dev(engine, nodes, N, st);
+    return;
+  }
+  switch (D) {
+    case 1: commit_cols_t<1>(engine, cw, N, M, nodes, st); break;
+    case 2: commit_cols_t<2>(engine, cw, N, M, nodes, st); break;
+    case 3: commit_cols_t<3>(engine, cw, N, M, nodes, st); break;
+    default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
+  }
+  merkle_from(engine, nodes, N, N / 4, st);
 }
 
 // ------------------------------------------------------------------ PoW

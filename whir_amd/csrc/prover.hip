@@ -174,8 +174,7 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec,
   }
   size_t nn = merkle_num_nodes(N);
   w.nodes = DevBuf(32 * nn, s);
-  hash_rows_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s);
-  merkle_build_dev(engine, w.nodes.as<uint8_t>(), N, s);
+  commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s);
   uint8_t root[32];
   download_sync(root, w.nodes.as<uint8_t>() + 32 * (nn - 1), 32, s);
   t.prover_hash(root);
