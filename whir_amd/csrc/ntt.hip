@@ -26,6 +26,7 @@
 #include "gl.h"
 #include "ntt.h"
 
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -187,6 +188,15 @@ static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipS
 // writes it back in place; that is exactly the state after log2(R) radix-2 DIF
 // stages, so the steps compose to the plain bit-reversed DIF.
 // --------------------------------------------------------------------------
+// LDS tiles are padded by one element per 32 (element e lives at e + e/32): the
+// strided butterfly accesses, the transposed tile writes and the bit-reversed reads
+// then spread over the banks instead of piling onto a few (up to 64-way conflicts
+// unpadded, about the minimum for the access width padded).
+template <bool PAD>
+__device__ __forceinline__ uint32_t lidx(uint32_t e) { return PAD ? e + (e >> 5) : e; }
+template <bool PAD>
+__host__ __device__ __forceinline__ size_t padded(size_t n) { return PAD ? n + (n >> 5) : n; }
+
 template <int D>
 __device__ __forceinline__ Ext<D> rot(const Ext<D>& a, unsigned k) {
   Ext<D> r;
@@ -240,7 +250,7 @@ __device__ __forceinline__ int brev_small(int m) {
   return ((m & 1) << 2) | (m & 2) | (m >> 2);
 }
 
-template <int D, int R, int LOGR>
+template <int D, bool PAD, int R, int LOGR>
 __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS, int logSub, int cols) {
   const int S = 1 << logS;
   const int stride = 1 << (logSub - LOGR);                 // S'/R
@@ -249,33 +259,33 @@ __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS
   for (int q = threadIdx.x; q < groups; q += blockDim.x) {
     const int base = q & (stride - 1);
     const int blk = q >> (logSub - LOGR);                  // col * (S/S') + block within column
-    Ext<D>* p = s + ((size_t)blk << logSub) + base;
+    const uint32_t e0 = ((uint32_t)blk << logSub) + base;
     Ext<D> y[R];
 #pragma unroll
-    for (int m = 0; m < R; m++) y[m] = p[m * stride];
+    for (int m = 0; m < R; m++) y[m] = s[lidx<PAD>(e0 + m * stride)];
     dif_regs<D, R>(y);
 #pragma unroll
     for (int m = 0; m < R; m++) {
       const int e = (base * brev_small<R>(m)) << tw_shift;
-      p[m * stride] = m == 0 ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
+      s[lidx<PAD>(e0 + m * stride)] = m == 0 ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
     }
   }
   (void)S;
 }
 
-template <int D>
+template <int D, bool PAD>
 __device__ __forceinline__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS, int cols) {
   int logSub = logS;
   while (logSub >= 3) {
-    dif_step<D, 8, 3>(s, tw, logS, logSub, cols);
+    dif_step<D, PAD, 8, 3>(s, tw, logS, logSub, cols);
     __syncthreads();
     logSub -= 3;
   }
   if (logSub == 2) {
-    dif_step<D, 4, 2>(s, tw, logS, logSub, cols);
+    dif_step<D, PAD, 4, 2>(s, tw, logS, logSub, cols);
     __syncthreads();
   } else if (logSub == 1) {
-    dif_step<D, 2, 1>(s, tw, logS, logSub, cols);
+    dif_step<D, PAD, 2, 1>(s, tw, logS, logSub, cols);
     __syncthreads();
   }
 }
@@ -315,15 +325,17 @@ __device__ __forceinline__ Ext<D> coeff_at(const uint64_t* msg, size_t L, const 
 // Single pass: C = 2^logC <= 2^11.  Each block transforms G sequences.
 // pre[c*C + i] = w_N^(c i).
 // --------------------------------------------------------------------------
-template <int D>
+template <int D, bool TWL, bool PAD>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint64_t* pre, const uint64_t* twS,
                                                             int G) {
   extern __shared__ uint64_t smem[];
   const int logC = a.logC;
   const int C = 1 << logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + (size_t)G * C * D;
-  for (int k = threadIdx.x; k < C; k += blockDim.x) tws[k] = twS[k];
+  uint64_t* twl = smem + padded<PAD>((size_t)G * C) * D;
+  if (TWL)
+    for (int k = threadIdx.x; k < C; k += blockDim.x) twl[k] = twS[k];
+  const uint64_t* tws = TWL ? twl : twS;
   const size_t nseq = (size_t)a.M * a.Q;
   const size_t seq0 = (size_t)blockIdx.x * G;
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
@@ -336,16 +348,16 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
                       a.mask_len, i);
       if (c) v = gl::escale<D>(v, pre[c * C + i]);
     }
-    s[e] = v;
+    s[lidx<PAD>(e)] = v;
   }
   __syncthreads();
-  lds_dft<D>(s, tws, logC, G);
+  lds_dft<D, PAD>(s, tws, logC, G);
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
     int g = e >> logC, t = e & (C - 1);
     size_t seq = seq0 + g;
     if (seq >= nseq) continue;
     size_t j = seq / a.Q, c = seq % a.Q;
-    Ext<D> v = s[(g << logC) + bitrev(t, logC)];
+    Ext<D> v = s[lidx<PAD>(((uint32_t)g << logC) + bitrev(t, logC))];
     uint64_t* o = a.out + (j * a.N + c * C + t) * D;
 #pragma unroll
     for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -361,7 +373,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
 // row[c*C2 + i2] = w_N^(c C1 i2); post as in k_shape_table.
 // Writes Y (same shape as the codeword) at [j][c*C + t2*C1 + i1].
 // --------------------------------------------------------------------------
-template <int D, int EPT>
+template <int D, int EPT, bool TWL, bool PAD, bool LORD>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint64_t* row, const uint64_t* post,
                                                            const uint64_t* twS, uint64_t* Y, int logC1, int logb,
                                                            int jn) {
@@ -371,8 +383,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
   const size_t C = (size_t)1 << a.logC;
   const int tile = C2 << logb;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + (size_t)tile * D;
-  for (int k = threadIdx.x; k < C2; k += blockDim.x) tws[k] = twS[k];
+  uint64_t* twl = smem + padded<PAD>((size_t)tile) * D;
+  if (TWL)
+    for (int k = threadIdx.x; k < C2; k += blockDim.x) twl[k] = twS[k];
+  const uint64_t* tws = TWL ? twl : twS;
   const uint32_t ntile = (uint32_t)(C1 >> logb);
   const uint32_t njg = (uint32_t)((a.M + jn - 1) / jn);
   uint32_t lb;
@@ -389,7 +403,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
 #pragma unroll
     for (int k = 0; k < EPT; k++) {
       int e = threadIdx.x + k * NTT_THREADS;
-      if (e < tile) pf[k] = postc[((size_t)(e >> logb) << logC1) + i1_0 + (e & (b - 1))];
+      if (e < tile) pf[k] = postc[((size_t)(LORD ? bitrev(e >> logb, logC2) : (e >> logb)) << logC1) + i1_0 + (e & (b - 1))];
     }
   }
   for (size_t j = j0; j < j1; j++) {
@@ -399,19 +413,22 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
       int col = e & (b - 1), i2 = e >> logb;
       Ext<D> v = coeff_at<D>(msg, a.L, mask, a.mask_len, (size_t)(i1_0 + col) + ((size_t)i2 << logC1));
       if (c) v = gl::escale<D>(v, rowc[i2]);
-      s[(col << logC2) + i2] = v;
+      s[lidx<PAD>(((uint32_t)col << logC2) + i2)] = v;
     }
     __syncthreads();
-    lds_dft<D>(s, tws, logC2, b);
+    lds_dft<D, PAD>(s, tws, logC2, b);
     uint64_t* yj = Y + (j * a.N + c * C) * D;
     if constexpr (EPT > 0) {
 #pragma unroll
       for (int k = 0; k < EPT; k++) {
         int e = threadIdx.x + k * NTT_THREADS;
         if (e < tile) {
-          int col = e & (b - 1), t2 = e >> logb;
+          // LORD: rows in bit-reversed order, so lanes read the tile in LDS order
+          // (output row t2 = bitrev(i) sits at tile index col*C2 + i)
+          int col = e & (b - 1), i = e >> logb;
+          int t2 = LORD ? bitrev(i, logC2) : i, li = LORD ? i : bitrev(i, logC2);
           size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
-          Ext<D> v = gl::escale<D>(s[(col << logC2) + bitrev(t2, logC2)], pf[k]);
+          Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], pf[k]);
           uint64_t* o = yj + o_idx * D;
 #pragma unroll
           for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -419,9 +436,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
       }
     } else {
       for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-        int col = e & (b - 1), t2 = e >> logb;
+        int col = e & (b - 1), i = e >> logb;
+        int t2 = LORD ? bitrev(i, logC2) : i, li = LORD ? i : bitrev(i, logC2);
         size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
-        Ext<D> v = gl::escale<D>(s[(col << logC2) + bitrev(t2, logC2)], postc[o_idx]);
+        Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], postc[o_idx]);
         uint64_t* o = yj + o_idx * D;
 #pragma unroll
         for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -440,7 +458,7 @@ constexpr int pass1_ept() { return D == 2 ? 8 : 0; }
 // --------------------------------------------------------------------------
 // Pass 2: rows.  Block = (b consecutive t2, coset c, message j).
 // --------------------------------------------------------------------------
-template <int D>
+template <int D, bool TWL, bool PAD, bool LORD>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint64_t* twS, const uint64_t* Y,
                                                            int logC1, int logb) {
   extern __shared__ uint64_t smem[];
@@ -448,8 +466,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
   const int C1 = 1 << logC1, C2 = 1 << logC2, b = 1 << logb;
   const size_t C = (size_t)1 << a.logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + ((size_t)C1 << logb) * D;
-  for (int k = threadIdx.x; k < C1; k += blockDim.x) tws[k] = twS[k];
+  uint64_t* twl = smem + padded<PAD>((size_t)C1 << logb) * D;
+  if (TWL)
+    for (int k = threadIdx.x; k < C1; k += blockDim.x) twl[k] = twS[k];
+  const uint64_t* tws = TWL ? twl : twS;
   // logical block = (j * Q + c) * ntile + tile: neighbouring row tiles are adjacent
   const uint32_t ntile = (uint32_t)(C2 >> logb);
   uint32_t lb;
@@ -464,14 +484,15 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
     Ext<D> v;
 #pragma unroll
     for (int d = 0; d < D; d++) v.c[d] = src[d];
-    s[e] = v;
+    s[lidx<PAD>(e)] = v;
   }
   __syncthreads();
-  lds_dft<D>(s, tws, logC1, b);
+  lds_dft<D, PAD>(s, tws, logC1, b);
   uint64_t* oj = a.out + (j * a.N + c * C) * D;
   for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int row = e & (b - 1), t1 = e >> logb;
-    Ext<D> v = s[(row << logC1) + bitrev(t1, logC1)];
+    int row = e & (b - 1), i = e >> logb;
+    int t1 = LORD ? bitrev(i, logC1) : i, li = LORD ? i : bitrev(i, logC1);
+    Ext<D> v = s[lidx<PAD>(((uint32_t)row << logC1) + li)];
     uint64_t* o = oj + ((size_t)(t2_0 + row) + ((size_t)t1 << logC2)) * D;
 #pragma unroll
     for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -521,8 +542,29 @@ static int floor_log2(int x) {
 }
 
 
-template <int D, int EPT>
-static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
+// Per-degree variants of the in-LDS transforms, each a bit per degree D:
+//   TWL   twiddles staged in LDS (else read through the vector L1)
+//   PAD   tile padded one element per 32
+//   LORD  pass outputs read in LDS order (rows written bit-reversed)
+// The defaults are the fastest measured on MI355X; WHIR_NTT_VARIANT = "twl,pad,lord"
+// masks overrides them for tuning.
+struct NttVariant {
+  int twl = 0xE, pad = 0xC, lord = 0x0;  // MI355X: Fp unpadded; Fp2/Fp3 padded; twiddles in LDS
+};
+static const NttVariant& ntt_variant() {
+  static NttVariant v = [] {
+    NttVariant r;
+    if (const char* e = getenv("WHIR_NTT_VARIANT")) sscanf(e, "%i,%i,%i", &r.twl, &r.pad, &r.lord);
+    return r;
+  }();
+  return v;
+}
+static bool tw_in_lds(int D) { return (ntt_variant().twl >> D) & 1; }
+static bool pad_lds(int D) { return (ntt_variant().pad >> D) & 1; }
+static bool lds_order(int D) { return (ntt_variant().lord >> D) & 1; }
+
+template <int D, int EPT, bool TWL, bool PAD, bool LORD>
+static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
   const int logC = a.logC, logC2 = logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
   int logb = std::min(logC1, floor_log2(std::max(1, tile_words() / (C2 * D))));
@@ -534,11 +576,51 @@ static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, i
   const uint64_t base_blocks = (uint64_t)(C1 >> logb) * a.Q;
   int jn = 1;
   while (jn * 2 <= (int)a.M && base_blocks * ((a.M + jn * 2 - 1) / (jn * 2)) >= 2048) jn *= 2;
-  size_t smem = ((size_t)tile * D + C2) * 8;
+  size_t smem = (padded<PAD>((size_t)tile) * D + (TWL ? C2 : 0)) * 8;
   dim3 grid(xcd_grid(base_blocks * ((a.M + jn - 1) / jn)));
-  allow_lds(k_ntt_pass1<D, EPT>, smem);
+  allow_lds(k_ntt_pass1<D, EPT, TWL, PAD, LORD>, smem);
   ProfScope prof(kernel_name("k_ntt_pass1", D, EPT), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-  k_ntt_pass1<D, EPT><<<grid, NTT_THREADS, smem, st>>>(a, row, post, flat_twiddles(logC2), scratch, logC1, logb, jn);
+  k_ntt_pass1<D, EPT, TWL, PAD, LORD><<<grid, NTT_THREADS, smem, st>>>(a, row, post, flat_twiddles(logC2), scratch, logC1,
+                                                              logb, jn);
+  WHIR_HIP(hipGetLastError());
+}
+
+template <int D, int EPT>
+static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
+  const int v = (tw_in_lds(D) ? 4 : 0) | (pad_lds(D) ? 2 : 0) | (lds_order(D) ? 1 : 0);
+  switch (v) {
+    case 0: pass1_t<D, EPT, false, false, false>(a, post, scratch, logN, logC1, st); break;
+    case 1: pass1_t<D, EPT, false, false, true>(a, post, scratch, logN, logC1, st); break;
+    case 2: pass1_t<D, EPT, false, true, false>(a, post, scratch, logN, logC1, st); break;
+    case 3: pass1_t<D, EPT, false, true, true>(a, post, scratch, logN, logC1, st); break;
+    case 4: pass1_t<D, EPT, true, false, false>(a, post, scratch, logN, logC1, st); break;
+    case 5: pass1_t<D, EPT, true, false, true>(a, post, scratch, logN, logC1, st); break;
+    case 6: pass1_t<D, EPT, true, true, false>(a, post, scratch, logN, logC1, st); break;
+    default: pass1_t<D, EPT, true, true, true>(a, post, scratch, logN, logC1, st); break;
+  }
+}
+
+template <int D, bool TWL, bool PAD>
+static void single_t(RsArgs a, const uint64_t* pre, int G, size_t nseq, hipStream_t st) {
+  const int C = 1 << a.logC;
+  size_t smem = (padded<PAD>((size_t)G * C) * D + (TWL ? C : 0)) * 8;
+  unsigned blocks = (unsigned)((nseq + G - 1) / G);
+  allow_lds(k_ntt_single<D, TWL, PAD>, smem);
+  ProfScope prof(kernel_name("k_ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
+  k_ntt_single<D, TWL, PAD><<<blocks, NTT_THREADS, smem, st>>>(a, pre, flat_twiddles(a.logC), G);
+  WHIR_HIP(hipGetLastError());
+}
+
+template <int D, bool TWL, bool PAD, bool LORD>
+static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, hipStream_t st) {
+  const int logC2 = a.logC - logC1;
+  const int C1 = 1 << logC1, C2 = 1 << logC2;
+  int logb = std::min(logC2, floor_log2(std::max(1, tile_words() / (C1 * D))));
+  size_t smem = (padded<PAD>((size_t)C1 << logb) * D + (TWL ? C1 : 0)) * 8;
+  dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
+  allow_lds(k_ntt_pass2<D, TWL, PAD, LORD>, smem);
+  ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
+  k_ntt_pass2<D, TWL, PAD, LORD><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
   WHIR_HIP(hipGetLastError());
 }
 
@@ -552,30 +634,29 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     const uint64_t* pre = shape_table(0, logN, logC, 0, st);
     int C = 1 << logC;
     int G = 1 << std::max(0, floor_log2(std::max(1, TW / (C * D))));
-    const uint64_t* twS = flat_twiddles(logC);
-    size_t smem = ((size_t)G * C * D + C) * 8;
-    unsigned blocks = (unsigned)((nseq + G - 1) / G);
-    allow_lds(k_ntt_single<D>, smem);
-    ProfScope prof(kernel_name("k_ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-    k_ntt_single<D><<<blocks, NTT_THREADS, smem, st>>>(a, pre, twS, G);
-    WHIR_HIP(hipGetLastError());
+    const int v = (tw_in_lds(D) ? 2 : 0) | (pad_lds(D) ? 1 : 0);
+    switch (v) {
+      case 0: single_t<D, false, false>(a, pre, G, nseq, st); break;
+      case 1: single_t<D, false, true>(a, pre, G, nseq, st); break;
+      case 2: single_t<D, true, false>(a, pre, G, nseq, st); break;
+      default: single_t<D, true, true>(a, pre, G, nseq, st); break;
+    }
     return;
   }
   WHIR_REQUIRE(logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
-  const int logC1 = (logC + 1) / 2, logC2 = logC - logC1;
-  const int C1 = 1 << logC1, C2 = 1 << logC2;
+  const int logC1 = (logC + 1) / 2;
   const uint64_t* post = shape_table(1, logN, logC, logC1, st);
-  {
-    pass1<D, pass1_ept<D>()>(a, post, scratch, logN, logC1, st);
-  }
-  {
-    int logb = std::min(logC2, floor_log2(std::max(1, TW / (C1 * D))));
-    size_t smem = (((size_t)C1 << logb) * D + C1) * 8;
-    dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
-    allow_lds(k_ntt_pass2<D>, smem);
-    ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
-    k_ntt_pass2<D><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
-    WHIR_HIP(hipGetLastError());
+  pass1<D, pass1_ept<D>()>(a, post, scratch, logN, logC1, st);
+  const int v = (tw_in_lds(D) ? 4 : 0) | (pad_lds(D) ? 2 : 0) | (lds_order(D) ? 1 : 0);
+  switch (v) {
+    case 0: pass2_t<D, false, false, false>(a, scratch, logC1, st); break;
+    case 1: pass2_t<D, false, false, true>(a, scratch, logC1, st); break;
+    case 2: pass2_t<D, false, true, false>(a, scratch, logC1, st); break;
+    case 3: pass2_t<D, false, true, true>(a, scratch, logC1, st); break;
+    case 4: pass2_t<D, true, false, false>(a, scratch, logC1, st); break;
+    case 5: pass2_t<D, true, false, true>(a, scratch, logC1, st); break;
+    case 6: pass2_t<D, true, true, false>(a, scratch, logC1, st); break;
+    default: pass2_t<D, true, true, true>(a, scratch, logC1, st); break;
   }
 }
 
