@@ -361,6 +361,86 @@ __global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3(const uint64_t*
   }
 }
 
+// Wave-cooperative variant for N >= 256: a wave takes 256 consecutive rows, lane l
+// hashing rows R0 + 64k + l (k < 4), so every row load is coalesced across the wave.
+// The two levels above are then shared out so every lane keeps working: lane l
+// makes the level-1 nodes of pair l/2 in groups l%2 and l%2 + 2, and the level-2
+// node of group l%4, quad m = l/4; the children come from lanes l^1 and l^2
+// through DPP quad permutations (no LDS).
+__device__ __forceinline__ void dpp_xor1(const uint32_t in[8], uint32_t out[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)in[i], 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ void dpp_xor2(const uint32_t in[8], uint32_t out[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)in[i], 0x4E, 0xF, 0xF, false);
+}
+__device__ __forceinline__ void sel8(bool c, const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = c ? a[i] : b[i];
+}
+
+template <int D>
+__global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3w(const uint64_t* __restrict__ cw, size_t N, size_t M,
+                                                                  uint8_t* __restrict__ nodes) {
+  const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+  const size_t R0 = wave * 256;
+  if (R0 >= N) return;  // whole waves only (N is a multiple of 256)
+  const int l = threadIdx.x & 63;
+  const size_t nb = M * D / 8;
+  uint32_t leaf[4][8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const size_t r = R0 + 64 * k + l;
+    uint32_t cv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) cv[i] = hd::B3_IV[i];
+#pragma unroll 1
+    for (size_t b = 0; b < nb; b++) {
+      uint32_t m[16];
+      row_block<D>(cw, N, r, b, m);
+      const uint32_t flags =
+          (b == 0 ? hd::B3_CHUNK_START : 0u) | (b + 1 == nb ? (hd::B3_CHUNK_END | hd::B3_ROOT) : 0u);
+      hd::b3_compress(cv, m, 64, 0, flags);
+    }
+    store_digest(nodes + 32 * r, cv);
+#pragma unroll
+    for (int i = 0; i < 8; i++) leaf[k][i] = cv[i];
+  }
+  // Level 1: pair i = l/2 of group k is (lane 2i, lane 2i + 1).  Even lanes make
+  // groups 0 and 2 (own leaf on the left), odd lanes groups 1 and 3 (own on the right).
+  const bool odd = l & 1;
+  uint32_t send[8], x1[8], x2[8], pa[8], pb[8];
+  sel8(odd, leaf[0], leaf[1], send);
+  dpp_xor1(send, x1);  // even: partner's leaf[0]; odd: partner's leaf[1]
+  sel8(odd, leaf[2], leaf[3], send);
+  dpp_xor1(send, x2);  // even: partner's leaf[2]; odd: partner's leaf[3]
+  {
+    uint32_t own_a[8], own_b[8];
+    sel8(odd, leaf[1], leaf[0], own_a);
+    sel8(odd, leaf[3], leaf[2], own_b);
+    if (odd) {
+      b3_pair(x1, own_a, pa);
+      b3_pair(x2, own_b, pb);
+    } else {
+      b3_pair(own_a, x1, pa);
+      b3_pair(own_b, x2, pb);
+    }
+  }
+  const size_t i1 = l >> 1, ka = odd ? 1 : 0;
+  store_digest(nodes + 32 * (N + R0 / 2 + 32 * ka + i1), pa);
+  store_digest(nodes + 32 * (N + R0 / 2 + 32 * (ka + 2) + i1), pb);
+  // Level 2: lane 4m + q makes group q's node m from the level-1 nodes (q, 2m) and
+  // (q, 2m + 1), held by lanes 4m + (q & 1) and 4m + 2 + (q & 1) (pa for q < 2, pb else).
+  const bool hi = (l >> 1) & 1;
+  uint32_t y[8], gp[8];
+  sel8(hi, pa, pb, send);
+  dpp_xor2(send, y);  // lanes with bit 1 clear receive the partner's pa, set: partner's pb
+  if (hi) b3_pair(y, pb, gp);
+  else b3_pair(pa, y, gp);
+  store_digest(nodes + 32 * (N + N / 2 + R0 / 4 + 16 * (l & 3) + (l >> 2)), gp);
+}
+
 static bool fused_tree_ok(int engine, size_t N, size_t W) {
   return engine == ENG_BLAKE3 && N >= 4 && (N & (N - 1)) == 0 && W % 8 == 0 && W > 0 && W <= 128;
 }
@@ -368,8 +448,13 @@ static bool fused_tree_ok(int engine, size_t N, size_t W) {
 template <int D>
 static void commit_cols_t(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st) {
   const size_t W = M * D;
-  ProfScope prof(kernel_name("k_leaves_tree_b3", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
-  k_leaves_tree_b3<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+  if (N >= 256) {
+    ProfScope prof(kernel_name("k_leaves_tree_b3w", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
+    k_leaves_tree_b3w<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+  } else {
+    ProfScope prof(kernel_name("k_leaves_tree_b3", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
+    k_leaves_tree_b3<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+  }
   WHIR_HIP(hipGetLastError());
 }
 
