@@ -276,6 +276,32 @@ void whir_witness_free(whir_witness *w);
 int whir_mle_evaluate_dev(int degree, const uint64_t *vector, size_t num_variables, const uint64_t *point,
                           uint64_t *out);
 
+/* ---- Batched and served proving.  No counterpart in the reference, which proves
+ * one instance per call (src/bin/main.rs:152-190); for callers that prove many
+ * instances or serve a stream of requests.  A pool of worker threads, each on its
+ * own HIP stream, runs whir_commit_dev + whir_prove_dev for queued jobs, starting
+ * the first queued job whose estimated peak device memory fits what the budget has
+ * left.  A job's proof is byte-identical to the same job run alone. */
+typedef struct whir_executor whir_executor;
+/* Estimated peak device bytes of one commit + prove under this configuration. */
+int whir_config_memory(const whir_config *cfg, size_t *bytes);
+/* hbm_budget 0: 80% of the device memory free at creation. */
+int whir_executor_new(int num_workers, size_t hbm_budget, whir_executor **out);
+/* Waits for every queued and running job, then stops the workers. */
+void whir_executor_free(whir_executor *e);
+/* kind 0: commit + prove (points and evaluations as for whir_prove_dev); kind 1:
+ * commit only.  cfg, t and the device vector must stay valid until the job has been
+ * waited for; points and evaluations are copied. */
+int whir_executor_submit(whir_executor *e, int kind, const whir_config *cfg, whir_transcript *t,
+                         const uint64_t *vector, const uint64_t *points, size_t num_points,
+                         const uint64_t *evaluations, uint64_t *job_id);
+/* Blocks until the job has finished and returns its status.  times (optional)
+ * receives {submitted, started, finished} on the executor's clock (seconds);
+ * eval_point (optional, num_variables * degree words) the folding randomness. */
+int whir_executor_wait(whir_executor *e, uint64_t job_id, double times[3], uint64_t *eval_point);
+/* Seconds since the executor was created (the clock of whir_executor_wait). */
+int whir_executor_clock(whir_executor *e, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,9 +112,15 @@ SIGNATURES = {
     "whir_commit_dev": [vp, vp, vp, ctypes.POINTER(vp)],
     "whir_prove_dev": [vp, vp, vp, vp, u64p, sz, u64p, u64p],
     "whir_witness_free": [vp],
+    "whir_config_memory": [vp, szp],
+    "whir_executor_new": [ci, sz, ctypes.POINTER(vp)],
+    "whir_executor_free": [vp],
+    "whir_executor_submit": [vp, ci, vp, vp, vp, u64p, sz, u64p, ctypes.POINTER(ctypes.c_uint64)],
+    "whir_executor_wait": [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), u64p],
+    "whir_executor_clock": [vp, ctypes.POINTER(ctypes.c_double)],
     "whir_mle_evaluate_dev": [ci, vp, sz, u64p, u64p],
 }
-VOID_FUNCS = {"whir_config_free", "whir_transcript_free", "whir_witness_free"}
+VOID_FUNCS = {"whir_config_free", "whir_transcript_free", "whir_witness_free", "whir_executor_free"}
 
 
 def lib():

@@ -1,0 +1,270 @@
+// Batched and served proving: a pool of worker threads, each proving on its own
+// HIP stream, that admits jobs by an HBM budget.
+//
+// The reference proves one instance per call (src/bin/main.rs:152-190 times one
+// commit + prove); this layer has no counterpart there.  Each WHIR proof is a
+// chain of ~50 Fiat-Shamir round trips between host and device, so a single
+// proof leaves the GPU idle between its rounds and small instances are bound by
+// launch and round-trip latency, not by the device.  Running many proofs at
+// once, each on its own stream, overlaps one proof's host work with the others'
+// kernels; the device interleaves their launches.  Every job uses the same
+// prover entry points as a plain call (whir_commit_dev + whir_prove_dev), so a
+// proof made in the mix is byte-identical to the same proof made alone.
+//
+// Admission: a job's peak device memory is estimated from its configuration
+// (job_memory); a worker starts the first queued job that fits in what the
+// budget has left, so small requests pass a large one that is waiting for
+// memory.  A job larger than the whole budget runs only when nothing else does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/whir_amd.h"
+#include "common.h"
+#include "config.h"
+#include "handles.h"
+
+using namespace whir;
+
+namespace whir {
+namespace {
+
+// Peak device bytes of one commit + prove: the covector and folded vector, the
+// initial codeword (plus its NTT scratch) and Merkle nodes, and the largest
+// round witness alive next to it.
+size_t job_memory(const WhirCfg& c) {
+  auto wit = [](const IrsCfg& irs, int Ds) {
+    const size_t N = irs.codeword_length, M = irs.num_messages();
+    return 2 * M * N * (size_t)Ds * 8 + 64 * N;
+  };
+  const size_t n = c.initial.vector_size;
+  size_t mx = 0;
+  for (const RoundCfg& r : c.rounds) mx = std::max(mx, wit(r.irs, c.D));
+  return 2 * n * (size_t)c.D * 8 + wit(c.initial, 1) + mx;
+}
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Job {
+  uint64_t id = 0;
+  int kind = 0;  // 0: commit + prove, 1: commit only
+  const whir_config* cfg = nullptr;
+  whir_transcript* t = nullptr;
+  const uint64_t* vector = nullptr;
+  std::vector<uint64_t> points, evals;
+  size_t np = 0;
+  size_t mem = 0;
+  // results
+  bool done = false;
+  int status = 0;
+  std::string error;
+  std::vector<uint64_t> eval_point;
+  double t_submit = 0, t_start = 0, t_end = 0;
+};
+
+}  // namespace
+}  // namespace whir
+
+struct whir_executor {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::vector<std::thread> workers;
+  size_t budget = 0, in_use = 0;
+  int running = 0;
+  uint64_t next_id = 1;
+  bool stop = false;
+  int device = 0;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  // First queued job that fits the remaining budget (or any job when idle).
+  Job* pick() {
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      Job* j = *it;
+      if (running == 0 || in_use + j->mem <= budget) {
+        queue.erase(it);
+        return j;
+      }
+    }
+    return nullptr;
+  }
+
+  void run(Job* j) {
+    int st = 0;
+    std::string err;
+    std::vector<uint64_t> ep;
+    whir_witness* w = nullptr;
+    st = whir_commit_dev(j->cfg, j->t, j->vector, &w);
+    if (st == 0 && j->kind == 0) {
+      const WhirCfg& c = j->cfg->cfg;
+      size_t rounds = c.initial_sumcheck.num_rounds + c.final_sumcheck.num_rounds;
+      for (const RoundCfg& r : c.rounds) rounds += r.sumcheck.num_rounds;
+      ep.assign(rounds * (size_t)c.D, 0);  // one folding challenge per variable
+      st = whir_prove_dev(j->cfg, j->t, j->vector, w, j->points.data(), j->np, j->evals.data(), ep.data());
+    } else if (st == 0) {
+      st = hipStreamSynchronize(thread_stream()) == hipSuccess ? 0 : 1;
+    }
+    if (st) err = whir_last_error();
+    if (w) whir_witness_free(w);
+    std::lock_guard<std::mutex> g(mu);
+    j->status = st;
+    j->error = err;
+    j->eval_point.swap(ep);
+  }
+
+  void loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || (j = pick()) != nullptr; });
+        if (!j) return;  // stop requested and nothing admissible left
+        in_use += j->mem;
+        running++;
+        j->t_start = now_s(t0);
+      }
+      run(j);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        j->t_end = now_s(t0);
+        j->done = true;
+        in_use -= j->mem;
+        running--;
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+#define EXEC_API_BEGIN try {
+#define EXEC_API_END                           \
+  }                                            \
+  catch (const std::exception& e) {            \
+    set_last_error(e.what());                  \
+    return 1;                                  \
+  }                                            \
+  catch (...) {                                \
+    set_last_error("unknown error");           \
+    return 1;                                  \
+  }                                            \
+  set_last_error("");                          \
+  return 0;
+
+int whir_config_memory(const whir_config* cfg, size_t* bytes) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(cfg && bytes, "config_memory: null argument");
+  *bytes = job_memory(cfg->cfg);
+  EXEC_API_END
+}
+
+int whir_executor_new(int num_workers, size_t hbm_budget, whir_executor** out) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(num_workers >= 1 && num_workers <= 64, "executor: 1..64 workers");
+  WHIR_REQUIRE(out, "executor: null output");
+  auto e = std::make_unique<whir_executor>();
+  WHIR_HIP(hipGetDevice(&e->device));
+  if (hbm_budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    WHIR_HIP(hipMemGetInfo(&free_b, &total_b));
+    hbm_budget = free_b / 10 * 8;  // leave a fifth of what is free for tables and other users
+  }
+  e->budget = hbm_budget;
+  whir_executor* p = e.get();
+  for (int i = 0; i < num_workers; i++) e->workers.emplace_back([p] { p->loop(); });
+  *out = e.release();
+  EXEC_API_END
+}
+
+void whir_executor_free(whir_executor* e) {
+  if (!e) return;
+  {
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->cv_done.wait(lk, [&] { return e->queue.empty() && e->running == 0; });
+    e->stop = true;
+  }
+  e->cv_work.notify_all();
+  for (auto& t : e->workers) t.join();
+  delete e;
+}
+
+int whir_executor_submit(whir_executor* e, int kind, const whir_config* cfg, whir_transcript* t,
+                         const uint64_t* vector, const uint64_t* points, size_t num_points,
+                         const uint64_t* evaluations, uint64_t* job_id) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(e && cfg && t && vector && job_id, "executor_submit: null argument");
+  WHIR_REQUIRE(kind == 0 || kind == 1, "executor_submit: kind is 0 (commit + prove) or 1 (commit)");
+  auto j = std::make_unique<Job>();
+  j->kind = kind;
+  j->cfg = cfg;
+  j->t = t;
+  j->vector = vector;
+  j->np = kind == 0 ? num_points : 0;
+  if (kind == 0) {
+    size_t nv = 0;
+    while (((size_t)1 << nv) < cfg->cfg.initial.vector_size) nv++;
+    const size_t D = (size_t)cfg->cfg.D;
+    if (num_points) WHIR_REQUIRE(points && evaluations, "executor_submit: null points or evaluations");
+    j->points.assign(points, points + num_points * nv * D);
+    j->evals.assign(evaluations, evaluations + num_points * D);
+  }
+  j->mem = job_memory(cfg->cfg);
+  {
+    std::lock_guard<std::mutex> g(e->mu);
+    WHIR_REQUIRE(!e->stop, "executor_submit: executor is shutting down");
+    j->id = e->next_id++;
+    j->t_submit = now_s(e->t0);
+    *job_id = j->id;
+    e->queue.push_back(j.get());
+    e->jobs.emplace(j->id, std::move(j));
+  }
+  e->cv_work.notify_one();
+  EXEC_API_END
+}
+
+int whir_executor_wait(whir_executor* e, uint64_t job_id, double times[3], uint64_t* eval_point) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(e, "executor_wait: null executor");
+  std::unique_ptr<Job> j;
+  {
+    std::unique_lock<std::mutex> lk(e->mu);
+    auto it = e->jobs.find(job_id);
+    WHIR_REQUIRE(it != e->jobs.end(), "executor_wait: unknown job id");
+    Job* p = it->second.get();
+    e->cv_done.wait(lk, [&] { return p->done; });
+    j = std::move(it->second);
+    e->jobs.erase(it);
+  }
+  if (times) {
+    times[0] = j->t_submit;
+    times[1] = j->t_start;
+    times[2] = j->t_end;
+  }
+  if (eval_point && !j->eval_point.empty()) std::copy(j->eval_point.begin(), j->eval_point.end(), eval_point);
+  if (j->status) throw Error("job " + std::to_string(job_id) + ": " + j->error);
+  EXEC_API_END
+}
+
+int whir_executor_clock(whir_executor* e, double* seconds) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(e && seconds, "executor_clock: null argument");
+  *seconds = now_s(e->t0);
+  EXEC_API_END
+}
+
+}  // extern "C"

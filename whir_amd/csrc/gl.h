@@ -59,13 +59,30 @@ GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
 }
 
 GL_FN uint64_t add(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Branch-free: one correction covers both cases.  If a + b wrapped, the sum is
+  // s + 2^64 = s + EPS (mod p) and s + EPS < p; if s >= p, s - p = s + EPS mod 2^64.
+  const uint64_t s = a + b, t = s + EPS;
+  return ((s < a) | (t < s)) ? t : s;
+#else
   uint64_t s = a + b;
   if (s < a) return s + EPS;            // wrapped past 2^64
   return s >= P ? s - P : s;
+#endif
 }
 GL_FN uint64_t sub(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // The borrow of the 64-bit subtraction selects d + p; taking it from the
+  // carry chain saves the separate compare.
+  uint32_t c0, c1;
+  const uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &c0);
+  const uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &c1);
+  const uint64_t d = ((uint64_t)hi << 32) | lo;
+  return c1 ? d + P : d;
+#else
   uint64_t d = a - b;
   return a < b ? d - EPS : d;           // d + p mod 2^64
+#endif
 }
 GL_FN uint64_t neg(uint64_t a) { return a ? P - a : 0; }
 

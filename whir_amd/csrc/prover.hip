@@ -28,17 +28,10 @@
 #include "hash.h"
 #include "ntt.h"
 #include "transcript.h"
+#include "handles.h"
 
 using namespace whir;
 
-struct whir_config {
-  WhirCfg cfg;
-};
-
-struct whir_transcript {
-  std::unique_ptr<Transcript> t;
-  Duplex* duplex = nullptr;  // set for the built-in transcript
-};
 
 namespace whir {
 

@@ -142,3 +142,74 @@ def mle_evaluate(degree: int, d_vector: DeviceBuffer, num_variables: int, point)
     out = np.zeros(degree, dtype=np.uint64)
     _call("whir_mle_evaluate_dev", degree, d_vector.ptr, num_variables, _p64(pt), _p64(out))
     return out
+
+
+def config_memory(cfg: Config) -> int:
+    """Estimated peak device bytes of one commit + prove under cfg."""
+    n = sz()
+    _call("whir_config_memory", cfg.h, ctypes.byref(n))
+    return n.value
+
+
+class Executor:
+    """Batched / served proving: `workers` threads, each on its own HIP stream, run
+    commit + prove (or commit only) for submitted jobs, admitted by an HBM budget
+    (bytes; 0 = 80% of the free device memory).  Results are byte-identical to the
+    same proofs made one at a time."""
+
+    COMMIT_AND_PROVE = 0
+    COMMIT_ONLY = 1
+
+    def __init__(self, workers: int = 8, hbm_budget: int = 0):
+        h = vp()
+        _call("whir_executor_new", workers, hbm_budget, ctypes.byref(h))
+        self.h = h.value
+        self._keep = {}
+
+    def submit(self, cfg: Config, t: Transcript, d_vector: DeviceBuffer, points=None, evaluations=None,
+               kind: int = COMMIT_AND_PROVE) -> int:
+        D, nv = cfg.degree, cfg.num_variables
+        pts = np.ascontiguousarray(np.asarray(points if points is not None else [], dtype=np.uint64)).reshape(-1)
+        ev = np.ascontiguousarray(np.asarray(evaluations if evaluations is not None else [],
+                                             dtype=np.uint64)).reshape(-1)
+        npts = ev.size // D
+        assert pts.size == npts * nv * D, "points must be num_points x num_variables x degree words"
+        pts = pts if pts.size else np.zeros(1, dtype=np.uint64)
+        ev = ev if ev.size else np.zeros(1, dtype=np.uint64)
+        jid = ctypes.c_uint64()
+        _call("whir_executor_submit", self.h, kind, cfg.h, t.h, d_vector.ptr, _p64(pts), npts, _p64(ev),
+              ctypes.byref(jid))
+        # the transcript, config and vector must outlive the job
+        self._keep[jid.value] = (cfg, t, d_vector, kind)
+        return jid.value
+
+    def wait(self, job_id: int):
+        """Blocks until the job ends; returns ((submitted, started, finished) seconds,
+        folding randomness (num_variables * degree words; empty for commit-only jobs)).
+        Raises WhirError if the job failed."""
+        times = (ctypes.c_double * 3)()
+        cfg, _, _, kind = self._keep[job_id]
+        words = cfg.num_variables * cfg.degree if kind == self.COMMIT_AND_PROVE else 0
+        ep = np.zeros(max(words, 1), dtype=np.uint64)
+        try:
+            _call("whir_executor_wait", self.h, job_id, times, _p64(ep) if words else None)
+        finally:
+            self._keep.pop(job_id, None)
+        return (times[0], times[1], times[2]), ep[:words]
+
+    def clock(self) -> float:
+        c = ctypes.c_double()
+        _call("whir_executor_clock", self.h, ctypes.byref(c))
+        return c.value
+
+    def close(self):
+        if self.h:
+            lib().whir_executor_free(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
