@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--num-variables", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-proving measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-qps", type=float, default=0.0, help="arrival rate; 0 = 70%% of measured capacity")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -153,6 +158,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(nv)
+    batched = None if args.no_batch else bench_batched(args.workers)
+    serving = None if args.no_serve else bench_serving(args.workers, args.serve_seconds, args.serve_qps)
 
     kernels = {k: {"launches": v["launches"], "ms_per_step": round(v["total_ms"] / args.steps, 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
@@ -178,9 +185,196 @@ def main():
         "verified_by_oracle_verifier": verified,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
         "kernels": kernels,
     }
     print(json.dumps(line))
+
+
+class _Instance:
+    """One PCS instance: device vector, evaluation points and claimed values."""
+
+    def __init__(self, cfg, dv, pts, evals, tag):
+        self.cfg, self.dv, self.pts, self.evals, self.tag = cfg, dv, pts, evals, tag
+
+    def transcript(self, GP, k=0):
+        import hashlib
+        pid = hashlib.sha3_512(json.dumps(self.cfg.describe(), sort_keys=True).encode()).digest()
+        return GP.Transcript(pid, hashlib.sha3_256(f"{self.tag} {k}".encode()).digest())
+
+
+def _config(GP, W, d, lir, cache):
+    key = (d, lir)
+    if key not in cache:
+        p = GP.ProtocolParameters(degree=2, security_level=100, pow_bits=20, initial_folding_factor=4,
+                                  folding_factor=4, unique_decoding=False, starting_log_inv_rate=lir,
+                                  hash_engine=W.HASH_BLAKE3)
+        cache[key] = GP.Config(1 << d, p).disable_pow()
+    return cache[key]
+
+
+def _points(GP, rng, dv, d, e):
+    pts = uniform_field(rng, e * d * 2)
+    evals = np.concatenate([GP.mle_evaluate(2, dv, d, pts[k * d * 2:(k + 1) * d * 2]) for k in range(e)])
+    return pts, evals
+
+
+def bench_batched(workers: int):
+    """64 independent PCS instances (Goldilocks2, Blake3, folding 4, rate 1/2,
+    num_variables ~ U{16..22}, 1-4 evaluation points, seeded), proved one after the
+    other on one stream (the plain path) and through the executor (`workers` proofs
+    in flight, one stream each).  Also the commit phase alone both ways."""
+    import whir_amd as W
+    from whir_amd import prover as GP
+    rng = np.random.default_rng(64)
+    cache = {}
+    insts = []
+    for k in range(64):
+        d = int(rng.integers(16, 23))
+        e = int(rng.integers(1, 5))
+        dv = W.DeviceBuffer.from_array(uniform_field(rng, 1 << d))
+        pts, evals = _points(GP, rng, dv, d, e)
+        insts.append(_Instance(_config(GP, W, d, 1, cache), dv, pts, evals, f"batched {k}"))
+
+    def plain(commit_only):
+        W.synchronize()
+        t0 = time.perf_counter()
+        out = []
+        for inst in insts:
+            t = inst.transcript(GP)
+            w = GP.commit(inst.cfg, t, inst.dv)
+            if not commit_only:
+                GP.prove(inst.cfg, t, inst.dv, w, inst.pts, inst.evals)
+            else:
+                W.synchronize()
+            out.append(t)
+        W.synchronize()
+        return time.perf_counter() - t0, out
+
+    def batched(commit_only):
+        ex = GP.Executor(workers=workers)
+        kind = GP.Executor.COMMIT_ONLY if commit_only else GP.Executor.COMMIT_AND_PROVE
+        try:
+            t0 = time.perf_counter()
+            jobs = [(ex.submit(i.cfg, t, i.dv, i.pts, i.evals, kind=kind), t)
+                    for i, t in ((i, i.transcript(GP)) for i in insts)]
+            for jid, _ in jobs:
+                ex.wait(jid)
+            dt = time.perf_counter() - t0
+        finally:
+            ex.close()
+        return dt, [t for _, t in jobs]
+
+    plain(False)  # warm-up: twiddle tables and buffer pools for every size
+    batched(False)
+    tp, ref = plain(False)
+    tb, got = batched(False)
+    tpc, _ = plain(True)
+    tbc, _ = batched(True)
+    identical = all(a.proof() == b.proof() for a, b in zip(ref, got))
+    sizes = sorted(int(i.cfg.num_variables) for i in insts)
+    return {"instances": len(insts), "workers": workers,
+            "num_variables": f"U{{16..22}} (seed 64), median {sizes[len(sizes) // 2]}",
+            "plain_proofs_per_s": round(len(insts) / tp, 1), "batched_proofs_per_s": round(len(insts) / tb, 1),
+            "proofs_speedup": round(tp / tb, 2),
+            "plain_commits_per_s": round(len(insts) / tpc, 1), "batched_commits_per_s": round(len(insts) / tbc, 1),
+            "commits_speedup": round(tpc / tbc, 2),
+            "identical_to_plain": identical,
+            "method": "executor: one HIP stream per worker, proofs in flight overlap their host round trips "
+                      "and kernels; plain = the same proofs one after another on one stream"}
+
+
+def bench_serving(workers: int, seconds: float, qps: float):
+    """Poisson arrivals of prove requests (Goldilocks2, Blake3, num_variables
+    log-uniform in [14, 24], 1-4 evaluations, rate 1/2 or 1/4) into the executor,
+    admitted by HBM budget; latency = finish - arrival.  With qps = 0 the rate is
+    70% of the throughput measured on a back-to-back burst of the same mix."""
+    import threading
+    import whir_amd as W
+    from whir_amd import prover as GP
+    rng = np.random.default_rng(5)
+    cache = {}
+    vecs = {d: W.DeviceBuffer.from_array(uniform_field(rng, 1 << d)) for d in range(14, 25)}
+    pools = {d: [_points(GP, rng, vecs[d], d, 1) for _ in range(4)] for d in range(14, 25)}
+
+    def request(k, r):
+        d = int(np.clip(np.floor(np.exp(r.uniform(np.log(14), np.log(25)))), 14, 24))
+        e = int(r.integers(1, 5))
+        lir = int(r.integers(1, 3))
+        sel = [pools[d][j] for j in r.permutation(4)[:e]]
+        pts = np.concatenate([p for p, _ in sel])
+        ev = np.concatenate([v for _, v in sel])
+        return _Instance(_config(GP, W, d, lir, cache), vecs[d], pts, ev, f"serve {k}")
+
+    def run(reqs, arrivals):
+        ex = GP.Executor(workers=workers)
+        lat, done = [], []
+        jobs = []
+        cv = threading.Condition()
+
+        def waiter():
+            k = 0
+            while True:
+                with cv:
+                    while k >= len(jobs):
+                        cv.wait()
+                    jid, t = jobs[k]
+                if jid is None:
+                    return
+                times, _ = ex.wait(jid)
+                lat.append(times[2] - times[0])
+                done.append((jid, t))
+                k += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        try:
+            t0 = ex.clock()
+            for inst, at in zip(reqs, arrivals):
+                delay = at - (ex.clock() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                t = inst.transcript(GP)
+                jid = ex.submit(inst.cfg, t, inst.dv, inst.pts, inst.evals)
+                with cv:
+                    jobs.append((jid, t))
+                    cv.notify()
+            with cv:
+                jobs.append((None, None))
+                cv.notify()
+            th.join()
+            span = ex.clock() - t0
+        finally:
+            ex.close()
+        return lat, span, done
+
+    r = np.random.default_rng(6)
+    burst = [request(k, r) for k in range(4 * workers)]
+    run(burst, [0.0] * len(burst))  # warm-up: tables and pools for every size
+    lat_b, span_b, _ = run(burst, [0.0] * len(burst))
+    capacity = len(burst) / span_b
+    rate = qps if qps > 0 else 0.7 * capacity
+    gaps = r.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < seconds]
+    reqs = [request(k, r) for k in range(len(arrivals))]
+    lat, span, done = run(reqs, arrivals)
+    lat_ms = np.sort(np.array(lat) * 1e3)
+    # the same request alone gives the same proof
+    same = True
+    for k in (0, len(reqs) // 2, len(reqs) - 1):
+        inst = reqs[k]
+        t = inst.transcript(GP)
+        w = GP.commit(inst.cfg, t, inst.dv)
+        GP.prove(inst.cfg, t, inst.dv, w, inst.pts, inst.evals)
+        same = same and t.proof() == done[k][1].proof()
+    return {"seconds": seconds, "arrival_qps": round(rate, 1), "burst_capacity_proofs_per_s": round(capacity, 1),
+            "requests": len(reqs), "throughput_proofs_per_s": round(len(reqs) / span, 1),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "mean_ms": round(float(lat_ms.mean()), 3), "workers": workers,
+            "mix": "num_variables log-uniform [14,24], 1-4 evaluations, log_inv_rate 1 or 2, Poisson arrivals",
+            "identical_alone": same}
 
 
 def cpu_baseline(nv_target: int):
