@@ -27,6 +27,9 @@ void sumcheck_poly_mixed_dev(int Ds, int D, const uint64_t* a, const uint64_t* b
                              hipStream_t st);
 size_t fold_into_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, size_t n, const uint64_t* w,
                      hipStream_t st);
+// fold_into_dev of a and fold_dev of b by the same w in one launch.
+size_t fold_both_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n, const uint64_t* w,
+                     hipStream_t st);
 size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n,
                                const uint64_t* w, uint64_t* dev_out, hipStream_t st);
 
@@ -46,8 +49,10 @@ void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_
 
 // mixed_univariate_evaluate at m host points x[0..m) in shared passes; results to
 // dev_out[0..m D).
+// padded: dev_out has room for m rounded up to a multiple of 4 values (the kernel
+// then writes its whole groups there directly).
 void univariate_eval_many_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, size_t m,
-                              uint64_t* dev_out, hipStream_t st);
+                              uint64_t* dev_out, hipStream_t st, bool padded = false);
 
 // mixed_univariate_evaluate (src/algebra/mod.rs:105-129): sum_i c_i x^i with c in the
 // base field (Ds = 1) or in the extension (Ds = D); result to dev_out[0..D).
@@ -65,6 +70,11 @@ void dot_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* de
 
 // out[k] = v[idx[k]] rows of a column-major matrix (gather for openings):
 // out[k*M + j] = cw[j*N + idx[k]] (degree-D elements).
+// Rows and Merkle sibling hashes of an opening in one launch: idx_sib = k row
+// indices then ns node indices (device); out_hashes 16-byte aligned.
+void gather_open_dev(int Ds, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
+                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t st);
+
 void gather_rows_dev(int D, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx, size_t k, uint64_t* out,
                      hipStream_t st);
 

@@ -81,6 +81,10 @@ static void finish_reduce(const uint64_t* parts, unsigned nparts, uint64_t* out,
 
 static unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(blocks_for(n, RT), MAX_BLOCKS); }
 
+// Up to this many element pairs a reducing pass runs as one block (four pairs per
+// lane), which stores its sums directly: cheaper than a second launch.
+static constexpr size_t SMALL_PAIRS = 1024;
+
 // Pass a small host constant (D words) to kernels by value.
 template <int D>
 static Ext<D> host_ext(const uint64_t* w) {
@@ -149,14 +153,15 @@ static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* ou
     return;
   }
   size_t half = next_pow2(n) >> 1, nhi = n - half;
-  unsigned g = grid_for(half);
-  DevBuf parts((size_t)g * 2 * D * 8, s);
+  // Small rounds: one block, which then writes the sums itself (no second launch).
+  const unsigned g = half <= SMALL_PAIRS ? 1u : grid_for(half);
+  DevBuf parts(g == 1 ? 0 : (size_t)g * 2 * D * 8, s);
   {
     ProfScope prof(kernel_name("k_sc_poly", Ds, D), s, 8.0 * (Ds + D) * (double)n);
-    k_sc_poly<Ds, D><<<g, RT, 0, s>>>(a, b, half, nhi, parts.as<uint64_t>());
+    k_sc_poly<Ds, D><<<g, RT, 0, s>>>(a, b, half, nhi, g == 1 ? out : parts.as<uint64_t>());
     WHIR_HIP(hipGetLastError());
   }
-  finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
+  if (g > 1) finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
 }
 
 void sumcheck_poly_mixed_dev(int Ds, int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out,
@@ -245,6 +250,49 @@ size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t s) 
   return fold_into_dev(D, D, v, v, n, w, s);
 }
 
+// Both sumcheck vectors folded by the same challenge in one launch (a from Ds words
+// into a_out, b in place).
+template <int Ds, int D>
+__global__ __launch_bounds__(RT) void k_fold2(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t half,
+                                              size_t nhi, Ext<D> w, Ext<D> one_minus_w) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x) {
+    st<D>(a_out, i, fold_at<Ds, D>(a_in, i, half, nhi, w, one_minus_w));
+    st<D>(b, i, fold_at<D, D>(b, i, half, nhi, w, one_minus_w));
+  }
+}
+
+template <int Ds, int D>
+static size_t fold2_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n, const uint64_t* w, hipStream_t s) {
+  if (n <= 1) {
+    fold_t<Ds, D>(a_in, a_out, n, w, s);
+    return n;
+  }
+  size_t half = next_pow2(n) >> 1, nhi = n - half;
+  Ext<D> ww = host_ext<D>(w);
+  Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
+  ProfScope prof(kernel_name("k_fold2", Ds, D), s, 8.0 * ((double)(Ds + D) * n + 2.0 * D * half));
+  k_fold2<Ds, D><<<grid_for(half), RT, 0, s>>>(a_in, a_out, b, half, nhi, ww, om);
+  WHIR_HIP(hipGetLastError());
+  return half;
+}
+
+size_t fold_both_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n, const uint64_t* w,
+                     hipStream_t s) {
+  if (Ds == D) {
+    switch (D) {
+      case 1: return fold2_t<1, 1>(a_in, a_out, b, n, w, s);
+      case 2: return fold2_t<2, 2>(a_in, a_out, b, n, w, s);
+      case 3: return fold2_t<3, 3>(a_in, a_out, b, n, w, s);
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: return fold2_t<1, 2>(a_in, a_out, b, n, w, s);
+      case 3: return fold2_t<1, 3>(a_in, a_out, b, n, w, s);
+    }
+  }
+  throw Error("fold: extension degree must be 1, 2 or 3");
+}
+
 // ------------------------------------------------------------------ fused fold + polynomial
 // Fold a (Ds words, into a_out) and b (in place) by w, then the round polynomial of
 // the folded pair.  Folded length h = next_pow2(n)/2 (a power of two >= 2); the new
@@ -279,14 +327,14 @@ static size_t fold_poly_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, si
   size_t h = next_pow2(n) >> 1, nhi = n - h;
   Ext<D> ww = host_ext<D>(w);
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
-  unsigned g = grid_for(h >> 1);
-  DevBuf parts((size_t)g * 2 * D * 8, s);
+  const unsigned g = (h >> 1) <= SMALL_PAIRS ? 1u : grid_for(h >> 1);
+  DevBuf parts(g == 1 ? 0 : (size_t)g * 2 * D * 8, s);
   {
     ProfScope prof(kernel_name("k_fold_poly", Ds, D), s, 8.0 * ((Ds + D) * (double)n + 2.0 * D * h));
-    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, parts.as<uint64_t>());
+    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, g == 1 ? out : parts.as<uint64_t>());
     WHIR_HIP(hipGetLastError());
   }
-  finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
+  if (g > 1) finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
   return h;
 }
 
@@ -588,7 +636,8 @@ __global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, const u
 }
 
 template <int Ds, int D>
-static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uint64_t* out, hipStream_t s) {
+static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uint64_t* out, hipStream_t s,
+                  bool padded) {
   for (size_t j0 = 0; j0 < m; j0 += UNI_MP) {
     const size_t mj = std::min<size_t>(UNI_MP, m - j0);
     uint64_t* o = out + j0 * D;
@@ -609,8 +658,9 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     upload_async(buf.ptr, xs + j0 * D, mj * D * 8, s);
     k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
     WHIR_HIP(hipGetLastError());
-    // The reduction writes all UNI_MP slots; only the first mj go to the caller.
-    uint64_t* dst = mj == UNI_MP ? o : res.as<uint64_t>();
+    // The reduction writes all UNI_MP slots; only the first mj go to the caller
+    // (straight into `out` when it has room for whole groups).
+    uint64_t* dst = (mj == UNI_MP || padded) ? o : res.as<uint64_t>();
     {
       ProfScope prof(kernel_name("k_uni", Ds, D), s, 8.0 * Ds * (double)n);
       k_uni<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, buf.as<uint64_t>(), L, (int)mj, parts.as<uint64_t>());
@@ -622,18 +672,18 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
 }
 
 void univariate_eval_many_dev(int Ds, int D, const uint64_t* c, size_t n, const uint64_t* xs, size_t m,
-                              uint64_t* out, hipStream_t s) {
+                              uint64_t* out, hipStream_t s, bool padded) {
   if (!m) return;
   if (Ds == 1) {
     switch (D) {
-      case 1: uni_t<1, 1>(c, n, xs, m, out, s); return;
-      case 2: uni_t<1, 2>(c, n, xs, m, out, s); return;
-      case 3: uni_t<1, 3>(c, n, xs, m, out, s); return;
+      case 1: uni_t<1, 1>(c, n, xs, m, out, s, padded); return;
+      case 2: uni_t<1, 2>(c, n, xs, m, out, s, padded); return;
+      case 3: uni_t<1, 3>(c, n, xs, m, out, s, padded); return;
     }
   } else if (Ds == D) {
     switch (D) {
-      case 2: uni_t<2, 2>(c, n, xs, m, out, s); return;
-      case 3: uni_t<3, 3>(c, n, xs, m, out, s); return;
+      case 2: uni_t<2, 2>(c, n, xs, m, out, s, padded); return;
+      case 3: uni_t<3, 3>(c, n, xs, m, out, s, padded); return;
     }
   }
   throw Error("univariate_evaluate: unsupported degrees");
@@ -714,6 +764,38 @@ __global__ void k_gather(const uint64_t* cw, size_t N, size_t M, const uint64_t*
   size_t r = idx[q];
   if (r >= N) return;
   st<D>(out, q * M + j, ld<D>(cw, j * N + r));
+}
+
+// Opening gather in one launch: rows (out_rows[q*M + j] = cw[j*N + idx[q]], Ds words
+// each) and Merkle sibling hashes (out_hashes[i] = nodes[sib[i]]); idx_sib holds the
+// k row indices followed by the ns node indices.
+template <int Ds>
+__global__ void k_gather_open(const uint64_t* __restrict__ cw, size_t N, size_t M, const uint64_t* __restrict__ idx_sib,
+                              size_t k, const uint4* __restrict__ nodes, size_t ns, uint64_t* out_rows,
+                              uint4* out_hashes) {
+  size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e < k * M) {
+    const size_t q = e / M, j = e % M, r = idx_sib[q];
+    if (r < N) st<Ds>(out_rows, e, ld<Ds>(cw, j * N + r));
+    return;
+  }
+  e -= k * M;
+  if (e < 2 * ns) out_hashes[e] = nodes[2 * idx_sib[k + (e >> 1)] + (e & 1)];
+}
+
+void gather_open_dev(int Ds, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
+                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t s) {
+  const size_t total = k * M + 2 * ns;
+  if (!total) return;
+  WHIR_REQUIRE(ns == 0 || ((uintptr_t)out_hashes & 15) == 0, "gather_open: hash output must be 16-byte aligned");
+  const unsigned g = blocks_for(total, RT);
+  switch (Ds) {
+    case 1: k_gather_open<1><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
+    case 2: k_gather_open<2><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
+    case 3: k_gather_open<3><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
+    default: throw Error("gather: extension degree must be 1, 2 or 3");
+  }
+  WHIR_HIP(hipGetLastError());
 }
 
 void gather_rows_dev(int D, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx, size_t k, uint64_t* out,

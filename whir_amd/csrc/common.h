@@ -44,7 +44,7 @@ void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Pinned host words (RESULT_MAILBOX_WORDS per stream) that kernels write small
 // results into directly, saving a device-to-host copy per result; read them after
 // synchronizing the stream.
-constexpr size_t RESULT_MAILBOX_WORDS = 4096;
+constexpr size_t RESULT_MAILBOX_WORDS = 65536;
 uint64_t* result_mailbox(hipStream_t s);
 
 // Device buffers recycled per stream: a buffer released on stream s goes to s's

@@ -21,7 +21,9 @@ void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t 
 
 // Leaf hashes and the whole Merkle tree of a column-major codeword into nodes
 // (merkle_num_nodes(N) hashes, laid out as merkle_build_dev leaves them).
-void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st);
+// The root also goes to root_out (32 bytes, device or mapped host memory) when given.
+void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                     uint8_t* root_out = nullptr);
 
 // Leaf hashes of a row-major matrix m[r*M + j].
 void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, uint8_t* leaves,
@@ -33,7 +35,7 @@ size_t merkle_layers(size_t num_leaves);
 // out[k] = nodes[idx[k]] (32-byte hashes), idx a device array of k node indices.
 void gather_hashes_dev(const uint8_t* nodes, const uint64_t* idx, size_t k, uint8_t* out, hipStream_t st);
 size_t merkle_num_nodes(size_t num_leaves);
-void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st);
+void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st, uint8_t* root_out = nullptr);
 
 // Proof of work: the minimal nonce in [first, first + count) with
 // u64_le(H(chal || nonce_le || 0^24)[0..8]) <= threshold; false if there is none.

@@ -222,7 +222,7 @@ __device__ __forceinline__ void reduce_levels(int engine, uint32_t (&h)[8][8], u
 // hashes the pair 2i, 2i + 1 from LDS), which keeps the chain of dependent hashes
 // short once few nodes are left.
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8_t* __restrict__ nodes, size_t W,
-                                                               size_t w, int L, int G1) {
+                                                               size_t w, int L, int G1, uint32_t* root_out) {
   __shared__ uint32_t buf[2][MERKLE_THREADS][8];
   size_t width = w;                                 // width of the current level
   size_t first = (size_t)blockIdx.x << L;           // block's first node within the current level
@@ -263,10 +263,17 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8
     cnt = lanes;
     left -= G;
   }
+  // The launch that makes the root also hands it to the caller (e.g. mapped host memory).
+  if (root_out && threadIdx.x < 8) root_out[threadIdx.x] = buf[cur][0][threadIdx.x];
 }
 
-// Merkle levels above a level of width w (a power of two) already in place.
-static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st) {
+// Merkle levels above a level of width w (a power of two) already in place.  The
+// root also goes to root_out (32 bytes) when given.
+static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st, uint8_t* root_out) {
+  if (w <= 1) {
+    if (root_out) WHIR_HIP(hipMemcpyAsync(root_out, nodes + 32 * (2 * W - 2), 32, hipMemcpyDefault, st));
+    return;
+  }
   while (w > 1) {
     // Wide levels: eight nodes per lane first (2^11 per block); otherwise pairs
     // (2^9 per block), which has the shortest chain of dependent hashes.
@@ -278,18 +285,19 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     const unsigned threads =
         (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
     ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)));
-    k_merkle_sub<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, L, G1);
+    k_merkle_sub<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, L, G1,
+                                                       blocks == 1 ? (uint32_t*)root_out : nullptr);
     WHIR_HIP(hipGetLastError());
     w >>= L;
   }
 }
 
-void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st) {
+void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st, uint8_t* root_out) {
   WHIR_REQUIRE(engine_supports_size(engine, 64), "merkle: engine does not hash 64-byte nodes");
   size_t layers = merkle_layers(num_leaves);
   size_t W = (size_t)1 << layers;
   if (W > num_leaves) fill_zero(nodes + 32 * num_leaves, 32 * (W - num_leaves), st);
-  merkle_from(engine, nodes, W, W, st);
+  merkle_from(engine, nodes, W, W, st, root_out);
 }
 
 // BLAKE3 leaves of a column-major codeword whose rows are single-chunk messages of
@@ -458,12 +466,13 @@ static void commit_cols_t(int engine, const uint64_t* cw, size_t N, size_t M, ui
   WHIR_HIP(hipGetLastError());
 }
 
-void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st) {
+void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                     uint8_t* root_out) {
   WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D), "leaf hash: engine does not support the row size");
   WHIR_REQUIRE(N > 0, "commit: empty codeword");
   if (!fused_tree_ok(engine, N, M * D)) {
     hash_rows_cols_dev(engine, D, cw, N, M, nodes, st);
-    merkle_build_dev(engine, nodes, N, st);
+    merkle_build_dev(engine, nodes, N, st, root_out);
     return;
   }
   switch (D) {
@@ -472,7 +481,7 @@ void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, 
     case 3: commit_cols_t<3>(engine, cw, N, M, nodes, st); break;
     default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
   }
-  merkle_from(engine, nodes, N, N / 4, st);
+  merkle_from(engine, nodes, N, N / 4, st, root_out);
 }
 
 // ------------------------------------------------------------------ PoW

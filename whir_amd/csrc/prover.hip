@@ -143,11 +143,6 @@ uint64_t coset_size(uint64_t masked_len, uint64_t N) {
   return c;
 }
 
-template <int D>
-void d2h(E<D>& out, const uint64_t* dev, hipStream_t s) {
-  download_sync(out.c, dev, D * 8, s);
-}
-
 // irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (one vector,
 // no masks).  vec holds vector_size elements of Ds words on the device.
 template <int D>
@@ -167,20 +162,23 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec,
   }
   size_t nn = merkle_num_nodes(N);
   w.nodes = DevBuf(32 * nn, s);
-  commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s);
+  // The root lands in the stream's pinned mailbox from the last tree launch.
+  uint64_t* mbox = result_mailbox(s);
+  commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
+  WHIR_HIP(hipStreamSynchronize(s));
   uint8_t root[32];
-  download_sync(root, w.nodes.as<uint8_t>() + 32 * (nn - 1), 32, s);
+  memcpy(root, mbox, 32);
   t.prover_hash(root);
   w.ood_points.clear();
   w.ood_values.clear();
   for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
   const size_t nood = c.out_domain_samples;
   if (nood) {
-    // All samples in shared passes over the vector, one read-back.
-    DevBuf out(nood * D * 8, s);
-    univariate_eval_many_dev(Ds, D, vec, c.vector_size, w.ood_points.data(), nood, out.as<uint64_t>(), s);
-    w.ood_values.resize(nood * D);
-    download_sync(w.ood_values.data(), out.ptr, nood * D * 8, s);
+    // All samples in shared passes over the vector, written to the mailbox.
+    WHIR_REQUIRE((nood + 3) / 4 * 4 * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain samples");
+    univariate_eval_many_dev(Ds, D, vec, c.vector_size, w.ood_points.data(), nood, mbox, s, true);
+    WHIR_HIP(hipStreamSynchronize(s));
+    w.ood_values.assign(mbox, mbox + nood * D);
     for (size_t i = 0; i < nood; i++) t.prover_field(D, &w.ood_values[i * D], 1);
   }
 }
@@ -196,20 +194,12 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
   uint64_t g = gl::root_of_unity(N);
   pts.clear();
   for (size_t i : idx) pts.push_back(gl::pow(g, (i / C) + (i % C) * Q));  // transpose_permute
-  rows.assign(k * M * w.Ds, 0);
-  if (k) {
-    std::vector<uint64_t> idx64(idx.begin(), idx.end());
-    DevBuf didx(k * 8, s), drows(k * M * w.Ds * 8, s);
-    upload_async(didx.ptr, idx64.data(), k * 8, s);
-    gather_rows_dev(w.Ds, w.cw.as<uint64_t>(), N, M, didx.as<uint64_t>(), k, drows.as<uint64_t>(), s);
-    download_sync(rows.data(), drows.ptr, k * M * w.Ds * 8, s);
-  }
-  t.hint_field_vec(w.Ds, rows.data(), k * M);
-  // merkle_tree::Config::open: sibling node indices in hint order
+  // merkle_tree::Config::open: sibling node indices in hint order (they depend on
+  // the indices alone, so rows and siblings are gathered in one launch).
   std::vector<size_t> cur(idx);
   std::sort(cur.begin(), cur.end());
   cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
-  std::vector<uint64_t> sib;
+  std::vector<uint64_t> idx_sib(idx.begin(), idx.end());
   size_t layers = merkle_layers(N), off = 0, width = (size_t)1 << layers;
   while (width > 1) {
     std::vector<size_t> next;
@@ -218,7 +208,7 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
       if (q + 1 < cur.size() && cur[q + 1] == (a ^ 1)) {
         q += 2;
       } else {
-        sib.push_back(off + (a ^ 1));
+        idx_sib.push_back(off + (a ^ 1));
         q += 1;
       }
       next.push_back(a >> 1);
@@ -227,14 +217,32 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
     off += width;
     width >>= 1;
   }
-  if (!sib.empty()) {
-    std::vector<uint8_t> hashes(32 * sib.size());
-    DevBuf dsib(sib.size() * 8, s), dh(32 * sib.size(), s);
-    upload_async(dsib.ptr, sib.data(), sib.size() * 8, s);
-    gather_hashes_dev(w.nodes.as<uint8_t>(), dsib.as<uint64_t>(), sib.size(), dh.as<uint8_t>(), s);
-    download_sync(hashes.data(), dh.ptr, hashes.size(), s);
-    for (size_t i = 0; i < sib.size(); i++) t.hint_hash(&hashes[32 * i]);
+  const size_t ns = idx_sib.size() - k;
+  const size_t row_words = k * M * w.Ds, hash_off = (row_words + 1) & ~(size_t)1;  // hashes 16-byte aligned
+  rows.assign(row_words, 0);
+  std::vector<uint8_t> hashes(32 * ns);
+  if (k + ns) {
+    DevBuf dis(idx_sib.size() * 8, s);
+    upload_async(dis.ptr, idx_sib.data(), idx_sib.size() * 8, s);
+    uint64_t* mbox = result_mailbox(s);
+    if (hash_off + 4 * ns <= RESULT_MAILBOX_WORDS) {
+      gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns, mbox,
+                      (uint8_t*)(mbox + hash_off), s);
+      WHIR_HIP(hipStreamSynchronize(s));
+      std::copy(mbox, mbox + row_words, rows.begin());
+      memcpy(hashes.data(), mbox + hash_off, hashes.size());
+    } else {
+      DevBuf dout((hash_off + 4 * ns) * 8, s);
+      gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns,
+                      dout.as<uint64_t>(), (uint8_t*)(dout.as<uint64_t>() + hash_off), s);
+      std::vector<uint64_t> h((hash_off + 4 * ns));
+      download_sync(h.data(), dout.ptr, h.size() * 8, s);
+      std::copy(h.begin(), h.begin() + row_words, rows.begin());
+      memcpy(hashes.data(), h.data() + hash_off, hashes.size());
+    }
   }
+  t.hint_field_vec(w.Ds, rows.data(), k * M);
+  for (size_t i = 0; i < ns; i++) t.hint_hash(&hashes[32 * i]);
 }
 
 // sumcheck::Config::prove (no masks), staggered fold + round polynomial.  The
@@ -273,9 +281,7 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     have_prev = true;
   }
   if (have_prev) {
-    size_t m = fold_into_dev(a_Ds, D, a_src, a, n, prev.c, s);
-    fold_dev(D, b, n, prev.c, s);
-    n = m;
+    n = fold_both_dev(a_Ds, D, a_src, a, b, n, prev.c, s);
   } else if (a_src != a) {
     lift_dev(D, a_src, n, a, s);
   }
