@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-proving measurement")
+    ap.add_argument("--no-other-configs", action="store_true", help="skip the d=20 Goldilocks2 and d=26 "
+                    "Goldilocks3 prover timings")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -160,6 +162,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(nv)
     pow_rates = bench_pow(W, GP, nv)
+    others = None if args.no_other_configs else bench_other_configs(W, GP)
     batched = None if args.no_batch else bench_batched(args.workers)
     serving = None if args.no_serve else bench_serving(args.workers, args.serve_seconds, args.serve_qps)
 
@@ -188,11 +191,47 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pow": pow_rates,
+        "other_configs": others,
         "batched": batched,
         "serving": serving,
         "kernels": kernels,
     }
     print(json.dumps(line))
+
+
+def bench_other_configs(W, GP, steps=3):
+    """Prover ms of the two other single-instance configurations: d=20 Goldilocks2
+    rate 1/2 Blake3, 1 evaluation point (PoW disabled as for the headline: its fold
+    PoW is 31-41 bits per round), and d=26 Goldilocks3 rate 1/2 SHA3 with the
+    configuration's own PoW (query-phase grinding, 16-20 bits), PCS with one point."""
+    import hashlib
+    out = {}
+    rng = np.random.default_rng(7)
+    for name, d, deg, lir, eng, e, dis in (("goldilocks2_d20_blake3", 20, 2, 1, W.HASH_BLAKE3, 1, True),
+                                           ("goldilocks3_d26_sha3_pow", 26, 3, 1, W.HASH_SHA3, 1, False)):
+        p = GP.ProtocolParameters(degree=deg, security_level=100, pow_bits=20, initial_folding_factor=4,
+                                  folding_factor=4, unique_decoding=False, starting_log_inv_rate=lir,
+                                  hash_engine=eng)
+        cfg = GP.Config(1 << d, p)
+        if dis:
+            cfg.disable_pow()
+        dv = W.DeviceBuffer.from_array(uniform_field(rng, 1 << d))
+        pts = uniform_field(rng, e * d * deg)
+        ev = np.concatenate([GP.mle_evaluate(deg, dv, d, pts[k * d * deg:(k + 1) * d * deg]) for k in range(e)])
+        pid = hashlib.sha3_512(json.dumps(cfg.describe(), sort_keys=True).encode()).digest()
+        times = []
+        for k in range(steps + 1):
+            t = GP.Transcript(pid, hashlib.sha3_256(f"{name} {k}".encode()).digest())
+            W.synchronize()
+            t0 = time.perf_counter()
+            w = GP.commit(cfg, t, dv)
+            GP.prove(cfg, t, dv, w, pts, ev)
+            W.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        out[name] = {"prover_ms": round(float(np.mean(times[1:])), 3), "pow": "disabled" if dis else "as configured",
+                     "proof_bytes": sum(len(x) for x in t.proof())}
+        del dv
+    return out
 
 
 def bench_pow(W, GP, nv):
