@@ -106,6 +106,14 @@ def _declare(L):
     L.oracle_bytes_to_indices.argtypes = [u8p, sz, sz, szp]
     L.oracle_pow_threshold.argtypes = [ctypes.c_double]
     L.oracle_pow_threshold.restype = ctypes.c_uint64
+    L.oracle_threads.argtypes = [ci]
+    L.oracle_threads.restype = ci
+
+
+def threads(n: int = 0) -> int:
+    """Set (n > 0) and return the oracle's OpenMP thread count; results do not
+    depend on it."""
+    return lib().oracle_threads(n)
 
 
 def _p64(a: np.ndarray):
@@ -272,7 +280,7 @@ def merkle_commit(engine: int, leaves: bytes) -> bytes:
 def merkle_open(nodes: bytes, num_leaves: int, indices) -> bytes:
     idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uintp))
     layers = layers_for_size(num_leaves)
-    buf = np.frombuffer(bytes(nodes), dtype=np.uint8).copy()
+    buf = np.frombuffer(nodes if isinstance(nodes, (bytes, bytearray)) else bytes(nodes), dtype=np.uint8)
     out = np.zeros(32 * max(1, len(idx) * max(layers, 1)), dtype=np.uint8)
     nh = lib().oracle_merkle_open(_p8(buf), num_leaves, idx.ctypes.data_as(szp), idx.size, _p8(out))
     return out[: 32 * nh].tobytes()

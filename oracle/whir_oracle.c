@@ -17,6 +17,7 @@
  *
  * Parity status per function: see the header of oracle/oracle.py.
  */
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -31,6 +32,14 @@ typedef unsigned __int128 u128;
 /* ------------------------------------------------------------------------- */
 
 #define GL_P 0xFFFFFFFF00000001ULL
+
+/* Threads of the OpenMP loops below (the large loops split into independent
+ * chunks; every partial sum is exact field arithmetic, so the results do not
+ * depend on the thread count).  n <= 0 keeps the current setting. */
+int oracle_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 static inline u64 f_add(u64 a, u64 b) { return (u64)(((u128)a + b) % GL_P); }
 static inline u64 f_sub(u64 a, u64 b) { return (u64)(((u128)a + GL_P - b) % GL_P); }
@@ -174,15 +183,30 @@ int oracle_evaluation_points(u64 masked_len, u64 N, const u64 *idx, size_t n, u6
 static void transpose_batch(int D, u64 *v, size_t len, size_t rows, size_t cols) {
     size_t sz = rows * cols;
     if (sz == 0 || len == 0) return;
-    u64 *tmp = (u64 *)malloc(sizeof(u64) * D * sz);
-    for (size_t m = 0; m < len / sz; m++) {
-        u64 *mat = v + m * sz * D;
+    size_t nm = len / sz;
+    if (nm == 1) { /* one large matrix: split its rows */
+        u64 *tmp = (u64 *)malloc(sizeof(u64) * D * sz);
+#pragma omp parallel for schedule(static) if (sz > 65536)
         for (size_t i = 0; i < rows; i++)
             for (size_t j = 0; j < cols; j++)
-                memcpy(tmp + (j * rows + i) * D, mat + (i * cols + j) * D, sizeof(u64) * D);
-        memcpy(mat, tmp, sizeof(u64) * D * sz);
+                memcpy(tmp + (j * rows + i) * D, v + (i * cols + j) * D, sizeof(u64) * D);
+        memcpy(v, tmp, sizeof(u64) * D * sz);
+        free(tmp);
+        return;
     }
-    free(tmp);
+#pragma omp parallel if (len > 65536)
+    {
+        u64 *tmp = (u64 *)malloc(sizeof(u64) * D * sz);
+#pragma omp for schedule(static)
+        for (size_t m = 0; m < nm; m++) {
+            u64 *mat = v + m * sz * D;
+            for (size_t i = 0; i < rows; i++)
+                for (size_t j = 0; j < cols; j++)
+                    memcpy(tmp + (j * rows + i) * D, mat + (i * cols + j) * D, sizeof(u64) * D);
+            memcpy(mat, tmp, sizeof(u64) * D * sz);
+        }
+        free(tmp);
+    }
 }
 
 /* apply_twiddles, cooley_tukey.rs:526-572: values[i][j] *= roots[(i*step*j) % len]
@@ -191,9 +215,10 @@ static void apply_twiddles(int D, u64 *v, size_t len, const u64 *roots, size_t r
                            size_t rows, size_t cols) {
     size_t size = rows * cols;
     size_t step = rlen / size;
+#pragma omp parallel for collapse(2) schedule(static) if (len > 65536)
     for (size_t m = 0; m < len / size; m++) {
-        u64 *mat = v + m * size * D;
         for (size_t i = 1; i < rows; i++) {
+            u64 *mat = v + m * size * D;
             size_t st = (i * step) % rlen;
             size_t index = st;
             for (size_t j = 1; j < cols; j++) {
@@ -281,9 +306,11 @@ static void ntt_dispatch(int D, const ntt_consts *k, u64 *v0, size_t len, const 
     case 1:
         return;
     case 2:
+#pragma omp parallel for schedule(static) if (len > 65536)
         for (size_t o = 0; o < len; o += 2) bfly(D, EL(v0, o), EL(v0, o + 1));
         return;
     case 3:
+#pragma omp parallel for schedule(static) if (len > 65536)
         for (size_t o = 0; o < len; o += 3) {
             u64 *v = EL(v0, o);
             u64 a0[3];
@@ -297,6 +324,7 @@ static void ntt_dispatch(int D, const ntt_consts *k, u64 *v0, size_t len, const 
         }
         return;
     case 4:
+#pragma omp parallel for schedule(static) if (len > 65536)
         for (size_t o = 0; o < len; o += 4) {
             u64 *v = EL(v0, o);
             bfly(D, EL(v, 0), EL(v, 2));
@@ -308,6 +336,7 @@ static void ntt_dispatch(int D, const ntt_consts *k, u64 *v0, size_t len, const 
         }
         return;
     case 8:
+#pragma omp parallel for schedule(static) if (len > 65536)
         for (size_t o = 0; o < len; o += 8) {
             u64 *v = EL(v0, o);
             bfly(D, EL(v, 0), EL(v, 4));
@@ -332,6 +361,7 @@ static void ntt_dispatch(int D, const ntt_consts *k, u64 *v0, size_t len, const 
         }
         return;
     case 16:
+#pragma omp parallel for schedule(static) if (len > 65536)
         for (size_t o = 0; o < len; o += 16) {
             u64 *v = EL(v0, o);
             for (int i = 0; i < 4; i++) {
@@ -688,9 +718,11 @@ static int hash_one(int engine, const u8 *in, size_t len, u8 out[32]) {
 /* HashEngine::hash_many, src/hash/mod.rs:79-85: `count` messages of `size`
  * bytes each, concatenated. */
 int oracle_hash_many(int engine, size_t size, const u8 *in, size_t count, u8 *out) {
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (count > 256)
     for (size_t i = 0; i < count; i++)
-        if (hash_one(engine, in + i * size, size, out + 32 * i)) return -1;
-    return 0;
+        bad |= hash_one(engine, in + i * size, size, out + 32 * i) != 0;
+    return bad ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -766,15 +798,20 @@ long oracle_merkle_open(const u8 *nodes, size_t num_leaves, const size_t *indice
  * leaf r = H(canonical LE bytes of row r). */
 int oracle_hash_rows(int engine, int D, const u64 *matrix, size_t rows, size_t cols, u8 *out) {
     size_t msg = 8 * (size_t)D * cols;
-    u8 *buf = (u8 *)malloc(msg ? msg : 1);
-    for (size_t r = 0; r < rows; r++) {
-        const u64 *row = matrix + r * cols * D;
-        for (size_t e = 0; e < cols * (size_t)D; e++)
-            for (int b = 0; b < 8; b++) buf[8 * e + b] = (u8)(row[e] >> (8 * b));
-        if (hash_one(engine, buf, msg, out + 32 * r)) { free(buf); return -1; }
+    int bad = 0;
+#pragma omp parallel reduction(| : bad) if (rows > 256)
+    {
+        u8 *buf = (u8 *)malloc(msg ? msg : 1);
+#pragma omp for schedule(static)
+        for (size_t r = 0; r < rows; r++) {
+            const u64 *row = matrix + r * cols * D;
+            for (size_t e = 0; e < cols * (size_t)D; e++)
+                for (int b = 0; b < 8; b++) buf[8 * e + b] = (u8)(row[e] >> (8 * b));
+            bad |= hash_one(engine, buf, msg, out + 32 * r) != 0;
+        }
+        free(buf);
     }
-    free(buf);
-    return 0;
+    return bad ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -799,23 +836,32 @@ void oracle_sumcheck_poly(int D, const u64 *a, size_t na, const u64 *b, size_t n
     }
     size_t half = next_pow2(n) >> 1;
     size_t nhi = n - half;
-    u64 t[3], u[3], v[3];
-    for (size_t i = 0; i < nhi; i++) {
-        e_mul(D, t, a + i * D, b + i * D);
-        e_add(D, c0, c0, t);
-        e_sub(D, u, a + (half + i) * D, a + i * D);
-        e_sub(D, v, b + (half + i) * D, b + i * D);
-        e_mul(D, t, u, v);
-        e_add(D, c2, c2, t);
+    /* partial sums per thread, added in any order (exact arithmetic) */
+#pragma omp parallel if (half > 4096)
+    {
+        u64 p0[3] = {0, 0, 0}, p2[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, t[3], u[3], v[3];
+#pragma omp for schedule(static) nowait
+        for (size_t i = 0; i < nhi; i++) {
+            e_mul(D, t, a + i * D, b + i * D);
+            e_add(D, p0, p0, t);
+            e_sub(D, u, a + (half + i) * D, a + i * D);
+            e_sub(D, v, b + (half + i) * D, b + i * D);
+            e_mul(D, t, u, v);
+            e_add(D, p2, p2, t);
+        }
+#pragma omp for schedule(static) nowait
+        for (size_t i = nhi; i < half; i++) {
+            e_mul(D, t, a + i * D, b + i * D);
+            e_add(D, acc, acc, t);
+        }
+        e_add(D, p0, p0, acc);
+        e_add(D, p2, p2, acc);
+#pragma omp critical
+        {
+            e_add(D, c0, c0, p0);
+            e_add(D, c2, c2, p2);
+        }
     }
-    u64 acc[3];
-    e_set_base(D, acc, 0);
-    for (size_t i = nhi; i < half; i++) {
-        e_mul(D, t, a + i * D, b + i * D);
-        e_add(D, acc, acc, t);
-    }
-    e_add(D, c0, c0, acc);
-    e_add(D, c2, c2, acc);
 }
 
 /* fold, src/algebra/sumcheck.rs:69-104.  Returns the new length. */
@@ -823,8 +869,9 @@ size_t oracle_fold(int D, u64 *v, size_t n, const u64 *w) {
     if (n <= 1) return n;
     size_t half = next_pow2(n) >> 1;
     size_t nhi = n - half;
-    u64 t[3], u[3];
+#pragma omp parallel for schedule(static) if (nhi > 4096)
     for (size_t i = 0; i < nhi; i++) {
+        u64 t[3], u[3];
         e_sub(D, u, v + (half + i) * D, v + i * D);
         e_mul(D, t, u, w);
         e_add(D, v + i * D, v + i * D, t);
@@ -850,7 +897,21 @@ static void eval_eq_rec(int D, u64 *acc, const u64 *point, size_t k, const u64 *
     eval_eq_rec(D, acc + half * D, point + D, k - 1, s1);
 }
 void oracle_eval_eq(int D, u64 *acc, const u64 *point, size_t k, const u64 *scalar) {
-    eval_eq_rec(D, acc, point, k, scalar);
+    /* the 2^t subtrees below the first t variables are independent */
+    size_t t = k < 6 ? 0 : 6;
+    size_t sub = (size_t)1 << (k - t);
+#pragma omp parallel for schedule(dynamic) if (t > 0)
+    for (size_t h = 0; h < ((size_t)1 << t); h++) {
+        u64 s[3];
+        memcpy(s, scalar, sizeof(u64) * D);
+        for (size_t q = 0; q < t; q++) {
+            u64 s1[3], s0[3];
+            e_mul(D, s1, s, point + q * D);
+            e_sub(D, s0, s, s1);
+            memcpy(s, ((h >> (t - 1 - q)) & 1) ? s1 : s0, sizeof(u64) * D);
+        }
+        eval_eq_rec(D, acc + h * sub * D, point + t * D, k - t, s);
+    }
 }
 
 /* multilinear_extend (exact size), src/algebra/multilinear.rs:6-151 */
@@ -870,31 +931,53 @@ void oracle_multilinear_extend(int D, const u64 *evals, size_t k, const u64 *poi
  * acc[i] += sum_j scalars[j] * points[j]^i. */
 void oracle_geometric_accumulate(int D, u64 *acc, size_t n, const u64 *scalars, const u64 *points,
                                  size_t m) {
-    u64 *s = (u64 *)malloc(sizeof(u64) * D * (m ? m : 1));
-    memcpy(s, scalars, sizeof(u64) * D * m);
-    for (size_t i = 0; i < n; i++) {
-        for (size_t j = 0; j < m; j++) {
-            e_add(D, acc + i * D, acc + i * D, s + j * D);
-            e_mul(D, s + j * D, s + j * D, points + j * D);
+    const size_t CH = 4096;
+#pragma omp parallel for schedule(dynamic) if (n > CH)
+    for (size_t i0 = 0; i0 < n; i0 += CH) {
+        size_t i1 = i0 + CH < n ? i0 + CH : n;
+        u64 *s = (u64 *)malloc(sizeof(u64) * D * (m ? m : 1));
+        for (size_t j = 0; j < m; j++) { /* s_j = scalar_j * x_j^i0 */
+            u64 xp[3];
+            e_pow(D, xp, points + j * D, i0);
+            e_mul(D, s + j * D, scalars + j * D, xp);
         }
+        for (size_t i = i0; i < i1; i++) {
+            for (size_t j = 0; j < m; j++) {
+                e_add(D, acc + i * D, acc + i * D, s + j * D);
+                e_mul(D, s + j * D, s + j * D, points + j * D);
+            }
+        }
+        free(s);
     }
-    free(s);
 }
 
 /* mixed_univariate_evaluate (Basefield embedding), src/algebra/mod.rs:105-129:
  * sum_i coeffs[i] * x^i with coeffs in the base field (Ds = 1) or the same
  * extension (Ds = D), x in the degree-D extension. */
 void oracle_univariate_evaluate(int Ds, int D, const u64 *coeffs, size_t n, const u64 *x, u64 *out) {
-    u64 acc[3];
-    e_set_base(D, acc, 0);
-    for (size_t i = n; i-- > 0;) {
-        e_mul(D, acc, acc, x);
-        u64 c[3];
-        if (Ds == 1) e_set_base(D, c, coeffs[i]);
-        else memcpy(c, coeffs + i * D, sizeof(u64) * D);
-        e_add(D, acc, acc, c);
+    /* Horner per chunk of 2^16 coefficients, chunk b scaled by x^(b 2^16) */
+    const size_t CH = (size_t)1 << 16;
+    u64 total[3];
+    e_set_base(D, total, 0);
+#pragma omp parallel for schedule(dynamic) if (n > CH)
+    for (size_t i0 = 0; i0 < n; i0 += CH) {
+        size_t i1 = i0 + CH < n ? i0 + CH : n;
+        u64 acc[3];
+        e_set_base(D, acc, 0);
+        for (size_t i = i1; i-- > i0;) {
+            e_mul(D, acc, acc, x);
+            u64 c[3];
+            if (Ds == 1) e_set_base(D, c, coeffs[i]);
+            else memcpy(c, coeffs + i * D, sizeof(u64) * D);
+            e_add(D, acc, acc, c);
+        }
+        u64 xp[3];
+        e_pow(D, xp, x, i0);
+        e_mul(D, acc, acc, xp);
+#pragma omp critical
+        e_add(D, total, total, acc);
     }
-    memcpy(out, acc, sizeof(u64) * D);
+    memcpy(out, total, sizeof(u64) * D);
 }
 
 /* lift (Basefield embedding map), src/algebra/mod.rs:51-59 */
@@ -913,13 +996,21 @@ void oracle_mixed_scalar_mul_add(int D, u64 *acc, const u64 *w, const u64 *v, si
 
 /* dot, src/algebra/mod.rs:36 */
 void oracle_dot(int D, const u64 *a, const u64 *b, size_t n, u64 *out) {
-    u64 acc[3], t[3];
-    e_set_base(D, acc, 0);
-    for (size_t i = 0; i < n; i++) {
-        e_mul(D, t, a + i * D, b + i * D);
-        e_add(D, acc, acc, t);
+    u64 total[3];
+    e_set_base(D, total, 0);
+#pragma omp parallel if (n > 4096)
+    {
+        u64 acc[3], t[3];
+        e_set_base(D, acc, 0);
+#pragma omp for schedule(static) nowait
+        for (size_t i = 0; i < n; i++) {
+            e_mul(D, t, a + i * D, b + i * D);
+            e_add(D, acc, acc, t);
+        }
+#pragma omp critical
+        e_add(D, total, total, acc);
     }
-    memcpy(out, acc, sizeof(u64) * D);
+    memcpy(out, total, sizeof(u64) * D);
 }
 
 /* ------------------------------------------------------------------------- */

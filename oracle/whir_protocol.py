@@ -541,9 +541,9 @@ def irs_open(t, cfg: IrsConfig, w: Witness):
     rows = [w.matrix[i].reshape(-1) for i in idx]
     sub = np.concatenate(rows) if rows else np.zeros(0, dtype=np.uint64)
     t.hint_field_vec(Ds, sub)
-    for h in [O.merkle_open(w.nodes, cfg.codeword_length, idx)[i:i + 32]
-              for i in range(0, len(O.merkle_open(w.nodes, cfg.codeword_length, idx)), 32)]:
-        t.hint_hash(h)
+    hints = O.merkle_open(w.nodes, cfg.codeword_length, idx)
+    for i in range(0, len(hints), 32):
+        t.hint_hash(hints[i:i + 32])
     return idx, pts, rows
 
 

@@ -473,3 +473,105 @@ def test_challenge_indices_mock_sponge_kats():
     assert two == [((sq[2 * i] << 8) | sq[2 * i + 1]) % (1 << 13) for i in range(5)]
     three = O.bytes_to_indices(sq, 5, 1 << 20)
     assert three == [int.from_bytes(sq[3 * i:3 * i + 3], "big") % (1 << 20) for i in range(5)]
+
+
+# ------------------------------------------------------------------ large sizes (OpenMP paths)
+# The oracle splits its large loops over threads (chunked Horner, chunked
+# geometric powers, split eq tree, partial sums).  These sizes take those paths;
+# each result is pinned by big-int arithmetic or a closed form, and must not
+# depend on the thread count.
+@pytest.fixture
+def oracle_threads():
+    before = O.threads()
+    yield
+    O.threads(before)
+
+
+@pytest.mark.parametrize("nth", [1, 8])
+def test_large_ntt_deltas_and_sampled_outputs(nth, oracle_threads):
+    O.threads(nth)
+    n = 1 << 17
+    w = O.ntt_root(n)
+    rng = random.Random(500)
+    v = np.array(rnd(rng, n), dtype=np.uint64)
+    got = O.ntt_batch(1, v, n)
+    vi = [int(x) for x in v]
+    for k in (0, 1, 12345, n - 1):
+        wk = pow(w, k, P)
+        acc = 0
+        for x in reversed(vi):
+            acc = (acc * wk + x) % P
+        assert int(got[k]) == acc
+    j = 777
+    d = np.zeros(n, dtype=np.uint64)
+    d[j] = 1
+    gd = O.ntt_batch(1, d, n)
+    for k in (0, 3, 99999, n - 1):
+        assert int(gd[k]) == pow(w, j * k, P)
+
+
+@pytest.mark.parametrize("nth", [1, 8])
+def test_large_hash_many_and_rows(nth, oracle_threads):
+    O.threads(nth)
+    rng = np.random.default_rng(501)
+    data = rng.integers(0, 256, size=1000 * 48, dtype=np.uint8).tobytes()
+    out = O.hash_many(O.ENGINE_SHA3, 48, data, 1000)
+    for i in (0, 1, 511, 999):
+        assert out[32 * i:32 * i + 32] == hashlib.sha3_256(data[48 * i:48 * i + 48]).digest()
+    m = rng.integers(0, P, size=(700, 6), dtype=np.uint64)
+    rows = O.hash_rows(O.ENGINE_SHA3, 2, m, 700, 3)
+    for r in (0, 350, 699):
+        assert rows[32 * r:32 * r + 32] == hashlib.sha3_256(m[r].astype("<u8").tobytes()).digest()
+
+
+@pytest.mark.parametrize("nth", [1, 8])
+def test_large_sumcheck_fold_dot(nth, oracle_threads):
+    O.threads(nth)
+    rng = random.Random(502)
+    n = 12000  # ragged: padded to 16384, half = 8192, 3808 upper entries
+    a, b = rnd(rng, n), rnd(rng, n)
+    c0, c2 = O.sumcheck_poly(1, a, b)
+    h = 8192
+    pa, pb = a + [0] * (2 * h - n), b + [0] * (2 * h - n)
+    assert int(c0[0]) == sum(pa[i] * pb[i] for i in range(h)) % P
+    assert int(c2[0]) == sum((pa[h + i] - pa[i]) * (pb[h + i] - pb[i]) for i in range(h)) % P
+    assert int(O.dot(1, a, b)[0]) == sum(x * y for x, y in zip(a, b)) % P
+    w = rng.randrange(P)
+    f = O.fold(1, a, [w])
+    assert f.size == h
+    for i in (0, 100, n - h - 1, n - h, h - 1):
+        assert int(f[i]) == (pa[i] + w * (pa[h + i] - pa[i])) % P
+
+
+@pytest.mark.parametrize("nth", [1, 8])
+def test_large_eval_eq_sampled(nth, oracle_threads):
+    O.threads(nth)
+    rng = random.Random(503)
+    k = 10
+    z, s = rnd(rng, k), rng.randrange(P)
+    got = O.eval_eq(1, np.zeros(1 << k, dtype=np.uint64), z, [s])
+    for x in (0, 1, 63, 64, 513, 1023):
+        e = s
+        for i in range(k):
+            e = e * (z[i] if (x >> (k - 1 - i)) & 1 else 1 - z[i]) % P
+        assert int(got[x]) == e
+
+
+@pytest.mark.parametrize("nth", [1, 8])
+def test_large_geometric_and_univariate(nth, oracle_threads):
+    O.threads(nth)
+    rng = random.Random(504)
+    n = 10000  # three chunks of 4096, the last ragged
+    xs, ss = rnd(rng, 3), rnd(rng, 3)
+    got = O.geometric_accumulate(1, [0] * n, ss, xs)
+    for i in (0, 4095, 4096, 8191, 8192, n - 1):
+        assert int(got[i]) == sum(s * pow(x, i, P) for s, x in zip(ss, xs)) % P
+    tot = sum(int(v) for v in O.geometric_accumulate(1, [0] * n, [1], [xs[0]])) % P
+    assert tot == (pow(xs[0], n, P) - 1) * pow(xs[0] - 1, P - 2, P) % P
+    m = (1 << 17) + 1234  # three Horner chunks of 2^16, the last ragged
+    c = rnd(rng, m)
+    x = rng.randrange(P)
+    acc = 0
+    for ci in reversed(c):
+        acc = (acc * x + ci) % P
+    assert int(O.univariate_evaluate(1, 1, c, [x])[0]) == acc
