@@ -212,6 +212,63 @@ __device__ __forceinline__ void bfly(Ext<D>& a, Ext<D>& b) {
   b = t;
 }
 
+// (a + b, b - a): a butterfly whose lower output is then multiplied by -2^k is the
+// same as this one followed by 2^k (2^96 = -1), which saves the negation.
+template <int D>
+__device__ __forceinline__ void bfly_neg(Ext<D>& a, Ext<D>& b) {
+  Ext<D> s = gl::eadd<D>(a, b), t = gl::esub<D>(b, a);
+  a = s;
+  b = t;
+}
+
+// 8-point DIF on y[0..7] (slot m ends with X[bitrev3(m)]).  w_8 = 2^120 = -2^24,
+// w_8^2 = w_4 = 2^48, w_8^3 = 2^168 = -2^72.
+template <int D>
+__device__ __forceinline__ void dif8(Ext<D>* y) {
+  bfly<D>(y[0], y[4]);
+  bfly_neg<D>(y[1], y[5]);
+  bfly<D>(y[2], y[6]);
+  bfly_neg<D>(y[3], y[7]);
+  y[5] = rot<D>(y[5], 24);
+  y[6] = rot<D>(y[6], 48);
+  y[7] = rot<D>(y[7], 72);
+  bfly<D>(y[0], y[2]);
+  bfly<D>(y[1], y[3]);
+  bfly<D>(y[4], y[6]);
+  bfly<D>(y[5], y[7]);
+  y[3] = rot<D>(y[3], 48);
+  y[7] = rot<D>(y[7], 48);
+  bfly<D>(y[0], y[1]);
+  bfly<D>(y[2], y[3]);
+  bfly<D>(y[4], y[5]);
+  bfly<D>(y[6], y[7]);
+}
+
+// 16-point DIF: one radix-2 level with w_16^m on the differences (w_16 = 2^156, the
+// reference's root: powers 2^{0,156,120,84,48,12,168,132}, the negative ones folded
+// into bfly_neg), then 8-point DIFs of both halves.  Slot s ends with X[bitrev4(s)]
+// (the even outputs come from the first half, the odd ones from the second).
+template <int D>
+__device__ __forceinline__ void dif16(Ext<D>* y) {
+  bfly<D>(y[0], y[8]);
+  bfly_neg<D>(y[1], y[9]);
+  bfly_neg<D>(y[2], y[10]);
+  bfly<D>(y[3], y[11]);
+  bfly<D>(y[4], y[12]);
+  bfly<D>(y[5], y[13]);
+  bfly_neg<D>(y[6], y[14]);
+  bfly_neg<D>(y[7], y[15]);
+  y[9] = rot<D>(y[9], 60);
+  y[10] = rot<D>(y[10], 24);
+  y[11] = rot<D>(y[11], 84);
+  y[12] = rot<D>(y[12], 48);
+  y[13] = rot<D>(y[13], 12);
+  y[14] = rot<D>(y[14], 72);
+  y[15] = rot<D>(y[15], 36);
+  dif8<D>(y);
+  dif8<D>(y + 8);
+}
+
 template <int D, int R>
 __device__ __forceinline__ void dif_regs(Ext<D> (&y)[R]) {
   if constexpr (R == 2) {
@@ -222,24 +279,10 @@ __device__ __forceinline__ void dif_regs(Ext<D> (&y)[R]) {
     y[3] = rot<D>(y[3], 48);  // w_4 = 2^48
     bfly<D>(y[0], y[1]);
     bfly<D>(y[2], y[3]);
+  } else if constexpr (R == 8) {
+    dif8<D>(y);
   } else {
-    bfly<D>(y[0], y[4]);
-    bfly<D>(y[1], y[5]);
-    bfly<D>(y[2], y[6]);
-    bfly<D>(y[3], y[7]);
-    y[5] = rot<D>(y[5], 120);  // w_8 = 2^120 = -2^24
-    y[6] = rot<D>(y[6], 48);   // w_8^2 = w_4
-    y[7] = rot<D>(y[7], 168);  // w_8^3 = 2^168
-    bfly<D>(y[0], y[2]);
-    bfly<D>(y[1], y[3]);
-    bfly<D>(y[4], y[6]);
-    bfly<D>(y[5], y[7]);
-    y[3] = rot<D>(y[3], 48);
-    y[7] = rot<D>(y[7], 48);
-    bfly<D>(y[0], y[1]);
-    bfly<D>(y[2], y[3]);
-    bfly<D>(y[4], y[5]);
-    bfly<D>(y[6], y[7]);
+    dif16<D>(y);
   }
 }
 
@@ -247,7 +290,8 @@ template <int R>
 __device__ __forceinline__ int brev_small(int m) {
   if constexpr (R == 2) return m;
   if constexpr (R == 4) return ((m & 1) << 1) | (m >> 1);
-  return ((m & 1) << 2) | (m & 2) | (m >> 2);
+  if constexpr (R == 8) return ((m & 1) << 2) | (m & 2) | (m >> 2);
+  return ((m & 1) << 3) | ((m & 2) << 1) | ((m & 4) >> 1) | (m >> 3);
 }
 
 template <int D, bool PAD, int R, int LOGR>
@@ -273,9 +317,18 @@ __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS
   (void)S;
 }
 
-template <int D, bool PAD>
+// R16: radix-16 steps first (4 stages per LDS round trip, 15/16 of a generic
+// multiply per element per four stages), then radix 8/4/2 for the rest.
+template <int D, bool PAD, bool R16>
 __device__ __forceinline__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS, int cols) {
   int logSub = logS;
+  if (R16) {
+    while (logSub >= 4) {
+      dif_step<D, PAD, 16, 4>(s, tw, logS, logSub, cols);
+      __syncthreads();
+      logSub -= 4;
+    }
+  }
   while (logSub >= 3) {
     dif_step<D, PAD, 8, 3>(s, tw, logS, logSub, cols);
     __syncthreads();
@@ -325,17 +378,15 @@ __device__ __forceinline__ Ext<D> coeff_at(const uint64_t* msg, size_t L, const 
 // Single pass: C = 2^logC <= 2^11.  Each block transforms G sequences.
 // pre[c*C + i] = w_N^(c i).
 // --------------------------------------------------------------------------
-template <int D, bool TWL, bool PAD>
+template <int D, bool PAD, bool R16>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint64_t* pre, const uint64_t* twS,
                                                             int G) {
   extern __shared__ uint64_t smem[];
   const int logC = a.logC;
   const int C = 1 << logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* twl = smem + padded<PAD>((size_t)G * C) * D;
-  if (TWL)
-    for (int k = threadIdx.x; k < C; k += blockDim.x) twl[k] = twS[k];
-  const uint64_t* tws = TWL ? twl : twS;
+  uint64_t* tws = smem + padded<PAD>((size_t)G * C) * D;
+  for (int k = threadIdx.x; k < C; k += blockDim.x) tws[k] = twS[k];
   const size_t nseq = (size_t)a.M * a.Q;
   const size_t seq0 = (size_t)blockIdx.x * G;
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
@@ -351,7 +402,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
     s[lidx<PAD>(e)] = v;
   }
   __syncthreads();
-  lds_dft<D, PAD>(s, tws, logC, G);
+  lds_dft<D, PAD, R16>(s, tws, logC, G);
   for (int e = threadIdx.x; e < G * C; e += blockDim.x) {
     int g = e >> logC, t = e & (C - 1);
     size_t seq = seq0 + g;
@@ -373,7 +424,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
 // row[c*C2 + i2] = w_N^(c C1 i2); post as in k_shape_table.
 // Writes Y (same shape as the codeword) at [j][c*C + t2*C1 + i1].
 // --------------------------------------------------------------------------
-template <int D, int EPT, bool TWL, bool PAD, bool LORD>
+template <int D, int EPT, bool PAD, bool R16>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint64_t* row, const uint64_t* post,
                                                            const uint64_t* twS, uint64_t* Y, int logC1, int logb,
                                                            int jn) {
@@ -383,10 +434,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
   const size_t C = (size_t)1 << a.logC;
   const int tile = C2 << logb;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* twl = smem + padded<PAD>((size_t)tile) * D;
-  if (TWL)
-    for (int k = threadIdx.x; k < C2; k += blockDim.x) twl[k] = twS[k];
-  const uint64_t* tws = TWL ? twl : twS;
+  uint64_t* tws = smem + padded<PAD>((size_t)tile) * D;
+  for (int k = threadIdx.x; k < C2; k += blockDim.x) tws[k] = twS[k];
   const uint32_t ntile = (uint32_t)(C1 >> logb);
   const uint32_t njg = (uint32_t)((a.M + jn - 1) / jn);
   uint32_t lb;
@@ -403,7 +452,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
 #pragma unroll
     for (int k = 0; k < EPT; k++) {
       int e = threadIdx.x + k * NTT_THREADS;
-      if (e < tile) pf[k] = postc[((size_t)(LORD ? bitrev(e >> logb, logC2) : (e >> logb)) << logC1) + i1_0 + (e & (b - 1))];
+      if (e < tile) pf[k] = postc[((size_t)(e >> logb) << logC1) + i1_0 + (e & (b - 1))];
     }
   }
   for (size_t j = j0; j < j1; j++) {
@@ -416,17 +465,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
       s[lidx<PAD>(((uint32_t)col << logC2) + i2)] = v;
     }
     __syncthreads();
-    lds_dft<D, PAD>(s, tws, logC2, b);
+    lds_dft<D, PAD, R16>(s, tws, logC2, b);
     uint64_t* yj = Y + (j * a.N + c * C) * D;
     if constexpr (EPT > 0) {
 #pragma unroll
       for (int k = 0; k < EPT; k++) {
         int e = threadIdx.x + k * NTT_THREADS;
         if (e < tile) {
-          // LORD: rows in bit-reversed order, so lanes read the tile in LDS order
-          // (output row t2 = bitrev(i) sits at tile index col*C2 + i)
-          int col = e & (b - 1), i = e >> logb;
-          int t2 = LORD ? bitrev(i, logC2) : i, li = LORD ? i : bitrev(i, logC2);
+          int col = e & (b - 1), t2 = e >> logb, li = bitrev(t2, logC2);
           size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
           Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], pf[k]);
           uint64_t* o = yj + o_idx * D;
@@ -436,8 +482,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
       }
     } else {
       for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-        int col = e & (b - 1), i = e >> logb;
-        int t2 = LORD ? bitrev(i, logC2) : i, li = LORD ? i : bitrev(i, logC2);
+        int col = e & (b - 1), t2 = e >> logb, li = bitrev(t2, logC2);
         size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
         Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], postc[o_idx]);
         uint64_t* o = yj + o_idx * D;
@@ -458,7 +503,7 @@ constexpr int pass1_ept() { return D == 2 ? 8 : 0; }
 // --------------------------------------------------------------------------
 // Pass 2: rows.  Block = (b consecutive t2, coset c, message j).
 // --------------------------------------------------------------------------
-template <int D, bool TWL, bool PAD, bool LORD>
+template <int D, bool PAD, bool R16>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint64_t* twS, const uint64_t* Y,
                                                            int logC1, int logb) {
   extern __shared__ uint64_t smem[];
@@ -466,10 +511,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
   const int C1 = 1 << logC1, C2 = 1 << logC2, b = 1 << logb;
   const size_t C = (size_t)1 << a.logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* twl = smem + padded<PAD>((size_t)C1 << logb) * D;
-  if (TWL)
-    for (int k = threadIdx.x; k < C1; k += blockDim.x) twl[k] = twS[k];
-  const uint64_t* tws = TWL ? twl : twS;
+  uint64_t* tws = smem + padded<PAD>((size_t)C1 << logb) * D;
+  for (int k = threadIdx.x; k < C1; k += blockDim.x) tws[k] = twS[k];
   // logical block = (j * Q + c) * ntile + tile: neighbouring row tiles are adjacent
   const uint32_t ntile = (uint32_t)(C2 >> logb);
   uint32_t lb;
@@ -487,11 +530,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
     s[lidx<PAD>(e)] = v;
   }
   __syncthreads();
-  lds_dft<D, PAD>(s, tws, logC1, b);
+  lds_dft<D, PAD, R16>(s, tws, logC1, b);
   uint64_t* oj = a.out + (j * a.N + c * C) * D;
   for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int row = e & (b - 1), i = e >> logb;
-    int t1 = LORD ? bitrev(i, logC1) : i, li = LORD ? i : bitrev(i, logC1);
+    int row = e & (b - 1), t1 = e >> logb, li = bitrev(t1, logC1);
     Ext<D> v = s[lidx<PAD>(((uint32_t)row << logC1) + li)];
     uint64_t* o = oj + ((size_t)(t2_0 + row) + ((size_t)t1 << logC2)) * D;
 #pragma unroll
@@ -542,28 +584,25 @@ static int floor_log2(int x) {
 }
 
 
-// Per-degree variants of the in-LDS transforms, each a bit per degree D:
-//   TWL   twiddles staged in LDS (else read through the vector L1)
-//   PAD   tile padded one element per 32
-//   LORD  pass outputs read in LDS order (rows written bit-reversed)
-// The defaults are the fastest measured on MI355X; WHIR_NTT_VARIANT = "twl,pad,lord"
+// Per-degree variants of the in-LDS transforms, one bit per degree D:
+//   pad  tile padded one element per 32
+//   r16  radix-16 register steps (else radix 8)
+// The defaults are the fastest measured on MI355X; WHIR_NTT_VARIANT = "pad,r16"
 // masks overrides them for tuning.
 struct NttVariant {
-  int twl = 0xE, pad = 0xC, lord = 0x0;  // MI355X: Fp unpadded; Fp2/Fp3 padded; twiddles in LDS
+  int pad = 0xC, r16 = 0x2;  // radix 16 pays only for D = 1 (D = 2 loses occupancy to registers)
 };
 static const NttVariant& ntt_variant() {
   static NttVariant v = [] {
     NttVariant r;
-    if (const char* e = getenv("WHIR_NTT_VARIANT")) sscanf(e, "%i,%i,%i", &r.twl, &r.pad, &r.lord);
+    if (const char* e = getenv("WHIR_NTT_VARIANT")) sscanf(e, "%i,%i", &r.pad, &r.r16);
     return r;
   }();
   return v;
 }
-static bool tw_in_lds(int D) { return (ntt_variant().twl >> D) & 1; }
-static bool pad_lds(int D) { return (ntt_variant().pad >> D) & 1; }
-static bool lds_order(int D) { return (ntt_variant().lord >> D) & 1; }
+static int variant_bits(int D) { return (((ntt_variant().pad >> D) & 1) << 1) | ((ntt_variant().r16 >> D) & 1); }
 
-template <int D, int EPT, bool TWL, bool PAD, bool LORD>
+template <int D, int EPT, bool PAD, bool R16>
 static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
   const int logC = a.logC, logC2 = logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
@@ -576,51 +615,46 @@ static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN,
   const uint64_t base_blocks = (uint64_t)(C1 >> logb) * a.Q;
   int jn = 1;
   while (jn * 2 <= (int)a.M && base_blocks * ((a.M + jn * 2 - 1) / (jn * 2)) >= 2048) jn *= 2;
-  size_t smem = (padded<PAD>((size_t)tile) * D + (TWL ? C2 : 0)) * 8;
+  size_t smem = (padded<PAD>((size_t)tile) * D + C2) * 8;
   dim3 grid(xcd_grid(base_blocks * ((a.M + jn - 1) / jn)));
-  allow_lds(k_ntt_pass1<D, EPT, TWL, PAD, LORD>, smem);
+  allow_lds(k_ntt_pass1<D, EPT, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_pass1", D, EPT), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-  k_ntt_pass1<D, EPT, TWL, PAD, LORD><<<grid, NTT_THREADS, smem, st>>>(a, row, post, flat_twiddles(logC2), scratch, logC1,
+  k_ntt_pass1<D, EPT, PAD, R16><<<grid, NTT_THREADS, smem, st>>>(a, row, post, flat_twiddles(logC2), scratch, logC1,
                                                               logb, jn);
   WHIR_HIP(hipGetLastError());
 }
 
 template <int D, int EPT>
 static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
-  const int v = (tw_in_lds(D) ? 4 : 0) | (pad_lds(D) ? 2 : 0) | (lds_order(D) ? 1 : 0);
-  switch (v) {
-    case 0: pass1_t<D, EPT, false, false, false>(a, post, scratch, logN, logC1, st); break;
-    case 1: pass1_t<D, EPT, false, false, true>(a, post, scratch, logN, logC1, st); break;
-    case 2: pass1_t<D, EPT, false, true, false>(a, post, scratch, logN, logC1, st); break;
-    case 3: pass1_t<D, EPT, false, true, true>(a, post, scratch, logN, logC1, st); break;
-    case 4: pass1_t<D, EPT, true, false, false>(a, post, scratch, logN, logC1, st); break;
-    case 5: pass1_t<D, EPT, true, false, true>(a, post, scratch, logN, logC1, st); break;
-    case 6: pass1_t<D, EPT, true, true, false>(a, post, scratch, logN, logC1, st); break;
-    default: pass1_t<D, EPT, true, true, true>(a, post, scratch, logN, logC1, st); break;
+  switch (variant_bits(D)) {
+    case 0: pass1_t<D, EPT, false, false>(a, post, scratch, logN, logC1, st); break;
+    case 1: pass1_t<D, EPT, false, true>(a, post, scratch, logN, logC1, st); break;
+    case 2: pass1_t<D, EPT, true, false>(a, post, scratch, logN, logC1, st); break;
+    default: pass1_t<D, EPT, true, true>(a, post, scratch, logN, logC1, st); break;
   }
 }
 
-template <int D, bool TWL, bool PAD>
+template <int D, bool PAD, bool R16>
 static void single_t(RsArgs a, const uint64_t* pre, int G, size_t nseq, hipStream_t st) {
   const int C = 1 << a.logC;
-  size_t smem = (padded<PAD>((size_t)G * C) * D + (TWL ? C : 0)) * 8;
+  size_t smem = (padded<PAD>((size_t)G * C) * D + C) * 8;
   unsigned blocks = (unsigned)((nseq + G - 1) / G);
-  allow_lds(k_ntt_single<D, TWL, PAD>, smem);
+  allow_lds(k_ntt_single<D, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
-  k_ntt_single<D, TWL, PAD><<<blocks, NTT_THREADS, smem, st>>>(a, pre, flat_twiddles(a.logC), G);
+  k_ntt_single<D, PAD, R16><<<blocks, NTT_THREADS, smem, st>>>(a, pre, flat_twiddles(a.logC), G);
   WHIR_HIP(hipGetLastError());
 }
 
-template <int D, bool TWL, bool PAD, bool LORD>
+template <int D, bool PAD, bool R16>
 static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, hipStream_t st) {
   const int logC2 = a.logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
   int logb = std::min(logC2, floor_log2(std::max(1, tile_words() / (C1 * D))));
-  size_t smem = (padded<PAD>((size_t)C1 << logb) * D + (TWL ? C1 : 0)) * 8;
+  size_t smem = (padded<PAD>((size_t)C1 << logb) * D + C1) * 8;
   dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
-  allow_lds(k_ntt_pass2<D, TWL, PAD, LORD>, smem);
+  allow_lds(k_ntt_pass2<D, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
-  k_ntt_pass2<D, TWL, PAD, LORD><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
+  k_ntt_pass2<D, PAD, R16><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
   WHIR_HIP(hipGetLastError());
 }
 
@@ -634,8 +668,7 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
     const uint64_t* pre = shape_table(0, logN, logC, 0, st);
     int C = 1 << logC;
     int G = 1 << std::max(0, floor_log2(std::max(1, TW / (C * D))));
-    const int v = (tw_in_lds(D) ? 2 : 0) | (pad_lds(D) ? 1 : 0);
-    switch (v) {
+    switch (variant_bits(D)) {
       case 0: single_t<D, false, false>(a, pre, G, nseq, st); break;
       case 1: single_t<D, false, true>(a, pre, G, nseq, st); break;
       case 2: single_t<D, true, false>(a, pre, G, nseq, st); break;
@@ -647,16 +680,11 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
   const int logC1 = (logC + 1) / 2;
   const uint64_t* post = shape_table(1, logN, logC, logC1, st);
   pass1<D, pass1_ept<D>()>(a, post, scratch, logN, logC1, st);
-  const int v = (tw_in_lds(D) ? 4 : 0) | (pad_lds(D) ? 2 : 0) | (lds_order(D) ? 1 : 0);
-  switch (v) {
-    case 0: pass2_t<D, false, false, false>(a, scratch, logC1, st); break;
-    case 1: pass2_t<D, false, false, true>(a, scratch, logC1, st); break;
-    case 2: pass2_t<D, false, true, false>(a, scratch, logC1, st); break;
-    case 3: pass2_t<D, false, true, true>(a, scratch, logC1, st); break;
-    case 4: pass2_t<D, true, false, false>(a, scratch, logC1, st); break;
-    case 5: pass2_t<D, true, false, true>(a, scratch, logC1, st); break;
-    case 6: pass2_t<D, true, true, false>(a, scratch, logC1, st); break;
-    default: pass2_t<D, true, true, true>(a, scratch, logC1, st); break;
+  switch (variant_bits(D)) {
+    case 0: pass2_t<D, false, false>(a, scratch, logC1, st); break;
+    case 1: pass2_t<D, false, true>(a, scratch, logC1, st); break;
+    case 2: pass2_t<D, true, false>(a, scratch, logC1, st); break;
+    default: pass2_t<D, true, true>(a, scratch, logC1, st); break;
   }
 }
 
