@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-other-configs", action="store_true", help="skip the d=20 Goldilocks2 and d=26 "
                     "Goldilocks3 prover timings")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-pow", action="store_true", help="skip the PoW grinding-rate measurement")
     ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="arrival rate; 0 = 70%% of measured capacity")
@@ -161,7 +162,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(nv, D, vec, points, evals, pid, sid, last.proof())
-    pow_rates = bench_pow(W, GP, nv)
+    pow_rates = None if args.no_pow else bench_pow(W, GP, nv)
     others = None if args.no_other_configs else bench_other_configs(W, GP)
     batched = None if args.no_batch else bench_batched(args.workers)
     serving = None if args.no_serve else bench_serving(args.workers, args.serve_seconds, args.serve_qps)

@@ -208,9 +208,10 @@ int whir_pow_grind_bounded(int engine, const uint8_t challenge[32], uint64_t thr
 /* ------------------------------------------------------------ WHIR prover
  * Replaces whir::Config::new (src/protocols/whir/config.rs:16-152),
  * whir::Config::commit (src/protocols/whir/mod.rs:79-92 -> irs_commit.rs:292-350)
- * and whir::Config::prove (src/protocols/whir/prover.rs:50-316) for one committed
- * base-field vector (batch_size 1, Basefield<Field64_D> embedding, or Identity
- * for degree 1) and MultilinearExtension linear forms (evaluation points). */
+ * and whir::Config::prove (src/protocols/whir/prover.rs:50-316) for base-field
+ * vectors (Basefield<Field64_D> embedding, or Identity for degree 1) committed
+ * batch_size at a time, one or more commitments proven together, and
+ * MultilinearExtension linear forms (evaluation points). */
 
 /* ProtocolParameters (src/parameters.rs:8-24) plus the field degree. */
 typedef struct whir_params {
@@ -221,7 +222,7 @@ typedef struct whir_params {
   int folding_factor;
   int unique_decoding;
   int starting_log_inv_rate;
-  int batch_size;             /* must be 1 */
+  int batch_size;             /* vectors per commitment (irs_commit.rs:58) */
   int hash_engine;            /* WHIR_HASH_* */
 } whir_params;
 
@@ -270,6 +271,21 @@ int whir_commit_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *
 int whir_prove_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *vector, const whir_witness *w,
                    const uint64_t *points, size_t num_points, const uint64_t *evaluations,
                    uint64_t *evaluation_point);
+/* whir::Config::commit of batch_size device-resident base-field vectors in one
+ * commitment (irs_commit.rs:292-350: vector b fills messages [b*2^k, (b+1)*2^k)
+ * of the committed matrix, OOD values point-major).  whir_commit_dev is the
+ * batch of one. */
+int whir_commit_batch_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *const *vectors,
+                          size_t num_vectors, whir_witness **out);
+/* whir::Config::prove over num_witnesses commitments holding num_vectors =
+ * num_witnesses * batch_size vectors, in commitment order (prover.rs:50-316:
+ * OOD cross terms, vector RLC, all initial witnesses opened at the first round).
+ * evaluations: num_points x num_vectors values (row-major), vector j at point k
+ * at (k * num_vectors + j) * degree.  whir_prove_dev is the case of one vector. */
+int whir_prove_batch_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *const *vectors,
+                         size_t num_vectors, const whir_witness *const *witnesses, size_t num_witnesses,
+                         const uint64_t *points, size_t num_points, const uint64_t *evaluations,
+                         uint64_t *evaluation_point);
 void whir_witness_free(whir_witness *w);
 /* MultilinearExtension::evaluate (mixed_multilinear_extend, src/algebra/multilinear.rs:15-151)
  * of a device base-field vector of 2^num_variables words at a degree-`degree` point. */

@@ -15,7 +15,8 @@ Restated from the reference (WizardOfMenlo/whir):
   geometric_challenge         src/protocols/geometric_challenge.rs:10-24
   sumcheck prove / verify     src/protocols/sumcheck.rs:69-216 (mask_length = 0)
   proof of work               src/protocols/proof_of_work.rs:68-192
-  prove                       src/protocols/whir/prover.rs:50-316 (one witness, MLE points)
+  prove                       src/protocols/whir/prover.rs:50-316 (MLE points; several
+                              witnesses of batch_size vectors each)
   verify                      src/protocols/whir/verifier.rs:45-264
 
 Transcript: the reference drives spongefish (git fcc277f8, not vendored here), whose
@@ -506,7 +507,8 @@ class Witness:
     matrix: np.ndarray   # row-major codeword_length x num_messages (elements of source_degree)
     nodes: bytes
     ood_points: list
-    ood_values: list
+    ood_values: list     # point-major: out_domain_samples x num_vectors
+    num_vectors: int = 1
 
 
 def irs_commit(t, cfg: IrsConfig, engine: int, vectors: list, Ds: int, D: int) -> Witness:
@@ -529,34 +531,54 @@ def irs_commit(t, cfg: IrsConfig, engine: int, vectors: list, Ds: int, D: int) -
             val = O.univariate_evaluate(Ds, D, ext(Ds, v), x)
             t.prover_field(D, val)
             vals.append(val)
-    return Witness(Ds, matrix, nodes, pts, vals)
+    return Witness(Ds, matrix, nodes, pts, vals, len(vectors))
 
 
 def irs_open(t, cfg: IrsConfig, w: Witness):
-    """irs_commit::Config::open, src/protocols/irs_commit.rs:385-432; returns (points, rows)."""
+    """irs_commit::Config::open, src/protocols/irs_commit.rs:385-432, for one witness;
+    returns (indices, points, rows)."""
+    return irs_open_many(t, cfg, [w])
+
+
+def irs_open_many(t, cfg: IrsConfig, witnesses: list):
+    """irs_commit::Config::open over several witnesses (irs_commit.rs:385-432): one
+    set of in-domain indices; per witness its opened rows and Merkle path as hints.
+    rows[q] is the horizontal concatenation of every witness's row idx[q]."""
     idx = challenge_indices(t, cfg.codeword_length, cfg.in_domain_samples, cfg.deduplicate_in_domain)
     pts = [int(x) for x in O.evaluation_points(cfg.masked_message_length(), cfg.codeword_length, idx)] if idx else []
-    cols = cfg.num_messages()
-    Ds = w.source_degree
-    rows = [w.matrix[i].reshape(-1) for i in idx]
-    sub = np.concatenate(rows) if rows else np.zeros(0, dtype=np.uint64)
-    t.hint_field_vec(Ds, sub)
-    hints = O.merkle_open(w.nodes, cfg.codeword_length, idx)
-    for i in range(0, len(hints), 32):
-        t.hint_hash(hints[i:i + 32])
+    per = []
+    for w in witnesses:
+        rows = [w.matrix[i].reshape(-1) for i in idx]
+        sub = np.concatenate(rows) if rows else np.zeros(0, dtype=np.uint64)
+        t.hint_field_vec(w.source_degree, sub)
+        hints = O.merkle_open(w.nodes, cfg.codeword_length, idx)
+        for i in range(0, len(hints), 32):
+            t.hint_hash(hints[i:i + 32])
+        per.append(rows)
+    rows = [np.concatenate([r[q] for r in per]) for q in range(len(idx))]
     return idx, pts, rows
 
 
 def irs_verify(r, cfg: IrsConfig, engine: int, root: bytes, Ds: int):
     """irs_commit::Config::verify, src/protocols/irs_commit.rs:438-484 (one commitment)."""
+    return irs_verify_many(r, cfg, engine, [root], Ds)
+
+
+def irs_verify_many(r, cfg: IrsConfig, engine: int, roots: list, Ds: int):
+    """irs_commit::Config::verify over several commitments (irs_commit.rs:438-484);
+    rows are horizontally concatenated across the commitments."""
     idx = challenge_indices(r, cfg.codeword_length, cfg.in_domain_samples, cfg.deduplicate_in_domain)
     pts = [int(x) for x in O.evaluation_points(cfg.masked_message_length(), cfg.codeword_length, idx)] if idx else []
-    sub = r.hint_field_vec(Ds)
     cols = cfg.num_messages()
-    vassert(sub.size == len(idx) * cols * Ds, "opened rows")
-    leaves = O.hash_rows(engine, Ds, sub, len(idx), cols) if idx else b""
-    merkle_verify(r, engine, cfg.codeword_length, root, idx, [leaves[32 * i:32 * i + 32] for i in range(len(idx))])
-    rows = [sub[k * cols * Ds:(k + 1) * cols * Ds] for k in range(len(idx))]
+    per = []
+    for root in roots:
+        sub = r.hint_field_vec(Ds)
+        vassert(sub.size == len(idx) * cols * Ds, "opened rows")
+        leaves = O.hash_rows(engine, Ds, sub, len(idx), cols) if idx else b""
+        merkle_verify(r, engine, cfg.codeword_length, root, idx,
+                      [leaves[32 * i:32 * i + 32] for i in range(len(idx))])
+        per.append([sub[k * cols * Ds:(k + 1) * cols * Ds] for k in range(len(idx))])
+    rows = [np.concatenate([p[k] for p in per]) for k in range(len(idx))]
     return pts, rows
 
 
@@ -635,7 +657,14 @@ def sumcheck_verify(r, cfg: SumcheckConfig, D, engine, the_sum):
 # ---------------------------------------------------------------- WHIR prove / verify
 def commit(t, cfg: WhirConfig, vector) -> Witness:
     """whir::Config::commit, src/protocols/whir/mod.rs:79-92 (base-field vector)."""
-    return irs_commit(t, cfg.initial, cfg.hash_engine, [ext(1, vector)], 1, cfg.D)
+    return commit_batch(t, cfg, [vector])
+
+
+def commit_batch(t, cfg: WhirConfig, vectors: list) -> Witness:
+    """whir::Config::commit of batch_size base-field vectors in one commitment
+    (src/protocols/whir/mod.rs:79-92 -> irs_commit.rs:292-350)."""
+    assert len(vectors) == cfg.initial.num_vectors, "commit: batch_size vectors per commitment"
+    return irs_commit(t, cfg.initial, cfg.hash_engine, [ext(1, v) for v in vectors], 1, cfg.D)
 
 
 def mle_evaluate_base(D, vector, point_list):
@@ -644,17 +673,56 @@ def mle_evaluate_base(D, vector, point_list):
     return O.multilinear_extend(D, lv, np.concatenate(point_list))
 
 
+def _row_dot(D, weights, row, Ds):
+    lrow = O.lift(D, row) if Ds == 1 else row
+    return O.dot(D, np.concatenate(weights), lrow)
+
+
+def _tensor(D, a, b):
+    """tensor_product, src/algebra/mod.rs:40-48."""
+    return [e_mul(D, x, y) for x in a for y in b]
+
+
 def prove(t, cfg: WhirConfig, vector, witness: Witness, points: list, evaluations: list):
     """whir::Config::prove, src/protocols/whir/prover.rs:50-316, with one committed
     vector and MultilinearExtension linear forms at `points` (each a list of
     num_variables extension elements)."""
+    return prove_batch(t, cfg, [vector], [witness], points, [[e] for e in evaluations])
+
+
+def prove_batch(t, cfg: WhirConfig, vectors: list, witnesses: list, points: list, evaluations: list,
+                check_invariants: bool = False):
+    """whir::Config::prove, src/protocols/whir/prover.rs:50-316: vectors committed in
+    `witnesses` (batch_size each, in order), MultilinearExtension linear forms at
+    `points`; evaluations[p][j] = MLE of vector j at point p.  check_invariants
+    asserts dot(vector, covector) == the_sum where the reference debug-asserts it."""
     D, eng = cfg.D, cfg.hash_engine
     n = cfg.initial.vector_size
-    # OOD rows of the single witness carry no cross terms.
-    oods_points = witness.ood_points
-    oods_values = witness.ood_values
-    vector_rlc = geometric_challenge(t, D, 1)
-    vec = O.lift(D, ext(1, vector))
+    V = len(vectors)
+    assert V == len(witnesses) * cfg.initial.num_vectors and V > 0
+    assert len(evaluations) == len(points) and all(len(row) == V for row in evaluations)
+    # Complete the OOD matrix with the cross terms (prover.rs:98-125): each
+    # witness's OOD rows hold its own vectors; the others are computed and sent.
+    oods_points, oods_matrix = [], []
+    off = 0
+    for w in witnesses:
+        B = w.num_vectors
+        for i, x in enumerate(w.ood_points):
+            row = []
+            for j in range(V):
+                if off <= j < off + B:
+                    row.append(w.ood_values[i * B + j - off])
+                else:
+                    val = O.univariate_evaluate(1, D, ext(1, vectors[j]), x)
+                    t.prover_field(D, val)
+                    row.append(val)
+            oods_points.append(x)
+            oods_matrix.append(row)
+        off += B
+    vector_rlc = geometric_challenge(t, D, V)
+    vec = O.lift(D, ext(1, vectors[0]))
+    for c, v in zip(vector_rlc[1:], vectors[1:]):
+        vec = O.mixed_scalar_mul_add(D, vec, c, ext(1, v))
     crlc = geometric_challenge(t, D, len(points) + len(oods_points))
     has_constraints = len(crlc) > 0
     form_rlc, oods_rlc = crlc[:len(points)], crlc[len(points):]
@@ -662,12 +730,14 @@ def prove(t, cfg: WhirConfig, vector, witness: Witness, points: list, evaluation
     for rc, pt in zip(form_rlc, points):
         cov = O.eval_eq(D, cov, np.concatenate(pt), rc)
     the_sum = np.zeros(D, dtype=np.uint64)
-    for rc, v in zip(form_rlc, evaluations):
-        the_sum = e_add(D, the_sum, e_mul(D, rc, e_mul(D, vector_rlc[0], v)))
+    for rc, row in zip(form_rlc, evaluations):
+        the_sum = e_add(D, the_sum, e_mul(D, rc, _row_dot(D, vector_rlc, np.concatenate(row), D)))
     if oods_points:
         cov = O.geometric_accumulate(D, cov, np.concatenate(oods_rlc), np.concatenate(oods_points))
-    for rc, v in zip(oods_rlc, oods_values):
-        the_sum = e_add(D, the_sum, e_mul(D, rc, e_mul(D, vector_rlc[0], v)))
+    for rc, row in zip(oods_rlc, oods_matrix):
+        the_sum = e_add(D, the_sum, e_mul(D, rc, _row_dot(D, vector_rlc, np.concatenate(row), D)))
+    if check_invariants and has_constraints:
+        assert np.array_equal(O.dot(D, vec, cov), the_sum), "dot(vector, covector) != the_sum"
     if has_constraints:
         vec, cov, the_sum, fr = sumcheck_prove(t, cfg.initial_sumcheck, D, eng, vec, cov, the_sum)
     else:
@@ -677,30 +747,32 @@ def prove(t, cfg: WhirConfig, vector, witness: Witness, points: list, evaluation
             vec = O.fold(D, vec, f)
         cov = np.zeros(cfg.initial_sumcheck.final_size() * D, dtype=np.uint64)
     evaluation_point = list(fr)
-    prev_cfg, prev_w = cfg.initial, witness
+    prev_cfg, prev_ws = cfg.initial, list(witnesses)
     for rc in cfg.rounds:
         new_w = irs_commit(t, rc.irs, eng, [vec], D, D)
         pow_prove(t, eng, rc.pow)
-        idx, pts, rows = irs_open(t, prev_cfg, prev_w)
-        w_eq = eq_weights(D, fr)  # vector_rlc = [1]
+        idx, pts, rows = irs_open_many(t, prev_cfg, prev_ws)
+        weights = _tensor(D, vector_rlc, list(eq_weights(D, fr)))
         stir_points = list(new_w.ood_points) + [O.lift(D, [p]) for p in pts]
         stir_vals = list(new_w.ood_values)
         for row in rows:
-            lrow = O.lift(D, row) if prev_w.source_degree == 1 else row
-            stir_vals.append(O.dot(D, w_eq.reshape(-1), lrow))
+            stir_vals.append(_row_dot(D, weights, row, prev_ws[0].source_degree))
         srlc = geometric_challenge(t, D, len(stir_points))
         if stir_points:
             cov = O.geometric_accumulate(D, cov, np.concatenate(srlc), np.concatenate(stir_points))
         for c, v in zip(srlc, stir_vals):
             the_sum = e_add(D, the_sum, e_mul(D, c, v))
+        if check_invariants:
+            assert np.array_equal(O.dot(D, vec, cov), the_sum), "dot(vector, covector) != the_sum"
         vec, cov, the_sum, fr = sumcheck_prove(t, rc.sumcheck, D, eng, vec, cov, the_sum)
         evaluation_point += fr
-        prev_cfg, prev_w = rc.irs, new_w
+        prev_cfg, prev_ws = rc.irs, [new_w]
+        vector_rlc = [e_one(D)]
     assert vec.size // D == cfg.final_sumcheck.initial_size
     for i in range(vec.size // D):
         t.prover_field(D, vec[i * D:(i + 1) * D])
     pow_prove(t, eng, cfg.final_pow)
-    irs_open(t, prev_cfg, prev_w)
+    irs_open_many(t, prev_cfg, prev_ws)
     vec, cov, the_sum, fr = sumcheck_prove(t, cfg.final_sumcheck, D, eng, vec, cov, the_sum)
     evaluation_point += fr
     return evaluation_point, form_rlc
@@ -710,19 +782,41 @@ def verify(r, cfg: WhirConfig, evaluations: list, points: list):
     """whir::Config::receive_commitment + verify + FinalClaim::verify
     (src/protocols/whir/mod.rs:63-74, 95-105; verifier.rs:45-264), one commitment,
     MultilinearExtension linear forms."""
+    return verify_batch(r, cfg, 1, [[e] for e in evaluations], points)
+
+
+def verify_batch(r, cfg: WhirConfig, num_commitments: int, evaluations: list, points: list):
+    """receive_commitment for each of num_commitments commitments (batch_size
+    vectors each), then whir::Config::verify (verifier.rs:45-264) and
+    FinalClaim::verify (whir/mod.rs:63-74) for MLE linear forms;
+    evaluations[p][j] as in prove_batch."""
     D, eng = cfg.D, cfg.hash_engine
-    root = r.prover_hash()
-    ood_pts = [r.verifier_field(D) for _ in range(cfg.initial.out_domain_samples)]
-    ood_vals = [r.prover_field(D) for _ in range(cfg.initial.out_domain_samples)]
-    vrlc = geometric_challenge(r, D, 1)
-    crlc = geometric_challenge(r, D, len(ood_pts) + len(evaluations))
+    B = cfg.initial.num_vectors
+    V = num_commitments * B
+    vassert(all(len(row) == V for row in evaluations), "evaluation matrix shape")
+    commits = []
+    for _ in range(num_commitments):
+        root = r.prover_hash()
+        opts = [r.verifier_field(D) for _ in range(cfg.initial.out_domain_samples)]
+        ovals = [r.prover_field(D) for _ in range(cfg.initial.out_domain_samples * B)]
+        commits.append((root, opts, ovals))
+    oods_points, oods_matrix = [], []
+    off = 0
+    for root, opts, ovals in commits:
+        for i, x in enumerate(opts):
+            row = [ovals[i * B + j - off] if off <= j < off + B else r.prover_field(D) for j in range(V)]
+            oods_points.append(x)
+            oods_matrix.append(row)
+        off += B
+    vrlc = geometric_challenge(r, D, V)
+    crlc = geometric_challenge(r, D, len(oods_points) + len(evaluations))
     form_rlc, oods_rlc = crlc[:len(evaluations)], crlc[len(evaluations):]
     the_sum = np.zeros(D, dtype=np.uint64)
-    for c, v in zip(form_rlc, evaluations):
-        the_sum = e_add(D, the_sum, e_mul(D, c, e_mul(D, vrlc[0], v)))
-    for c, v in zip(oods_rlc, ood_vals):
-        the_sum = e_add(D, the_sum, e_mul(D, c, e_mul(D, vrlc[0], v)))
-    round_constraints = [(oods_rlc, ood_pts)]
+    for c, row in zip(form_rlc, evaluations):
+        the_sum = e_add(D, the_sum, e_mul(D, c, _row_dot(D, vrlc, np.concatenate(row), D)))
+    for c, row in zip(oods_rlc, oods_matrix):
+        the_sum = e_add(D, the_sum, e_mul(D, c, _row_dot(D, vrlc, np.concatenate(row), D)))
+    round_constraints = [(oods_rlc, oods_points)]
     if not crlc:
         vassert(not the_sum.any())
         fr = [r.verifier_field(D) for _ in range(cfg.initial_sumcheck.num_rounds)]
@@ -730,34 +824,30 @@ def verify(r, cfg: WhirConfig, evaluations: list, points: list):
     else:
         the_sum, fr = sumcheck_verify(r, cfg.initial_sumcheck, D, eng, the_sum)
     round_fr = [fr]
-    prev = ("init", cfg.initial, root, 1)
+    prev = (cfg.initial, [c[0] for c in commits], 1, vrlc)
     for rc in cfg.rounds:
         nroot = r.prover_hash()
         npts = [r.verifier_field(D) for _ in range(rc.irs.out_domain_samples)]
         nvals = [r.prover_field(D) for _ in range(rc.irs.out_domain_samples)]
         pow_verify(r, eng, rc.pow)
-        pts, rows = irs_verify(r, prev[1], eng, prev[2], prev[3])
-        w_eq = eq_weights(D, round_fr[-1])
-        weights = list(npts) + [O.lift(D, [p]) for p in pts]
-        vals = list(nvals)
-        for row in rows:
-            lrow = O.lift(D, row) if prev[3] == 1 else row
-            vals.append(O.dot(D, w_eq.reshape(-1), lrow))
+        pts, rows = irs_verify_many(r, prev[0], eng, prev[1], prev[2])
+        weights = _tensor(D, prev[3], list(eq_weights(D, round_fr[-1])))
+        cw = list(npts) + [O.lift(D, [p]) for p in pts]
+        vals = list(nvals) + [_row_dot(D, weights, row, prev[2]) for row in rows]
         srlc = geometric_challenge(r, D, len(vals))
         for c, v in zip(srlc, vals):
             the_sum = e_add(D, the_sum, e_mul(D, c, v))
-        round_constraints.append((srlc, weights))
+        round_constraints.append((srlc, cw))
         the_sum, fr = sumcheck_verify(r, rc.sumcheck, D, eng, the_sum)
         round_fr.append(fr)
-        prev = ("round", rc.irs, nroot, D)
+        prev = (rc.irs, [nroot], D, [e_one(D)])
     nf = cfg.final_sumcheck.initial_size
     final_vector = r.prover_field(D, nf)
     pow_verify(r, eng, cfg.final_pow)
-    pts, rows = irs_verify(r, prev[1], eng, prev[2], prev[3])
-    w_eq = eq_weights(D, round_fr[-1])
+    pts, rows = irs_verify_many(r, prev[0], eng, prev[1], prev[2])
+    weights = _tensor(D, prev[3], list(eq_weights(D, round_fr[-1])))
     for p, row in zip(pts, rows):
-        lrow = O.lift(D, row) if prev[3] == 1 else row
-        val = O.dot(D, w_eq.reshape(-1), lrow)
+        val = _row_dot(D, weights, row, prev[2])
         ev = O.univariate_evaluate(D, D, final_vector, O.lift(D, [p]))
         vassert(np.array_equal(ev, val), "final in-domain")
     the_sum, ffr = sumcheck_verify(r, cfg.final_sumcheck, D, eng, the_sum)
@@ -767,11 +857,11 @@ def verify(r, cfg: WhirConfig, evaluations: list, points: list):
     inv = _e_inv(D, poly_eval)
     linear_form_rlc = e_mul(D, the_sum, inv)
     nv_list = [cfg.initial.vector_size.bit_length() - 1] + [rc.irs.vector_size.bit_length() - 1 for rc in cfg.rounds]
-    for ri, (coeffs, weights) in enumerate(round_constraints):
+    for ri, (coeffs, wts) in enumerate(round_constraints):
         nv = nv_list[ri]
         start = max(len(evaluation_point) - nv, 0)
         sub = evaluation_point[start:]
-        for c, x in zip(coeffs, weights):
+        for c, x in zip(coeffs, wts):
             linear_form_rlc = e_sub(D, linear_form_rlc, e_mul(D, c, univariate_mle(D, x, sub)))
     # FinalClaim::verify with MLE linear forms
     rlc = np.zeros(D, dtype=np.uint64)

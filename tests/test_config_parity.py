@@ -50,3 +50,15 @@ def test_config_matches_oracle(D, nv, ff, lir, ud, sp):
     cfg.disable_pow()
     ref.disable_pow()
     assert cfg.describe() == ref.describe()
+
+
+@pytest.mark.parametrize("B", [2, 3, 8])
+@pytest.mark.parametrize("D,nv", [(2, 16), (3, 20)])
+def test_config_batch_size_matches_oracle(B, D, nv):
+    """batch_size only widens the initial commitment (irs_commit.rs:181-190)."""
+    p = WP.ProtocolParameters(security_level=100, pow_bits=20, starting_log_inv_rate=1, batch_size=B)
+    ref = WP.WhirConfig.new(1 << nv, p, D)
+    cfg = WPROV.Config(1 << nv, WPROV.ProtocolParameters(degree=D, security_level=100, pow_bits=20,
+                                                         starting_log_inv_rate=1, batch_size=B))
+    assert cfg.describe() == ref.describe()
+    assert cfg.describe()["initial"]["num_vectors"] == B

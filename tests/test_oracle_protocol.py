@@ -151,3 +151,84 @@ def test_tampered_proofs_are_rejected():
     wrong = [WP.e_add(2, evals[0], WP.e_one(2))]
     with pytest.raises(WP.VerificationError):
         WP.verify(WP.TranscriptReader(pid, sid, narg, hints), cfg, wrong, points)
+
+
+# ---------------------------------------------------------------- several vectors / commitments
+def run_batch(nv, D, B, W, num_points, engine=O.ENGINE_BLAKE3, k0=2, k=2, lir=1, ud=False, seed=7):
+    """batch_size = B vectors per commitment, W commitments, all proven together
+    (whir/prover.rs:50-316 with vectors.len() = W * B; the reference's tests of
+    several witnesses, whir/mod.rs:346-894)."""
+    p = WP.ProtocolParameters(security_level=32, pow_bits=0, initial_folding_factor=k0, folding_factor=k,
+                              unique_decoding=ud, starting_log_inv_rate=lir, batch_size=B, hash_engine=engine)
+    cfg = WP.WhirConfig.new(1 << nv, p, D)
+    cfg.disable_pow()
+    pid, sid = WP.protocol_ids(f"batch nv={nv} D={D} B={B} W={W} e={engine}")
+    vecs = [rand_vec(seed + j, 1 << nv) for j in range(B * W)]
+    rng = random.Random(seed)
+    points = [[np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)]
+              for _ in range(num_points)]
+    evals = [[WP.mle_evaluate_base(D, v, pt) for v in vecs] for pt in points]
+    t = WP.Transcript(pid, sid)
+    ws = [WP.commit_batch(t, cfg, vecs[w * B:(w + 1) * B]) for w in range(W)]
+    ep, _ = WP.prove_batch(t, cfg, vecs, ws, points, evals, check_invariants=True)
+    r = WP.TranscriptReader(pid, sid, bytes(t.narg), bytes(t.hints))
+    vp = WP.verify_batch(r, cfg, W, evals, points)
+    assert len(vp) == len(ep) and all(np.array_equal(a, b) for a, b in zip(vp, ep))
+    return cfg, pid, sid, bytes(t.narg), bytes(t.hints), vecs, ws, evals, points
+
+
+@pytest.mark.parametrize("B,W", [(1, 2), (2, 1), (2, 2), (3, 1), (1, 3)])
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("num_points", [0, 1, 2])
+def test_batch_prove_verify_roundtrip(B, W, D, num_points):
+    run_batch(10, D, B, W, num_points)
+
+
+@pytest.mark.parametrize("B,W", [(2, 2), (1, 2)])
+def test_batch_unique_decoding_and_sha3(B, W):
+    run_batch(9, 2, B, W, 1, engine=O.ENGINE_SHA3, ud=True, k0=3, k=2)
+
+
+def test_batch_single_vector_is_the_plain_proof():
+    """One vector in one commitment: the batched prover is the plain prover."""
+    cfg, pid, sid, narg, hints, vecs, ws, evals, points = run_batch(10, 2, 1, 1, 2)
+    t = WP.Transcript(pid, sid)
+    w = WP.commit(t, cfg, vecs[0])
+    WP.prove(t, cfg, vecs[0], w, points, [row[0] for row in evals])
+    assert (bytes(t.narg), bytes(t.hints)) == (narg, hints)
+
+
+def test_batch_commit_columns_are_per_vector_blocks():
+    """messages = vectors.flat_map(chunks) (irs_commit.rs:317-320): vector b owns
+    columns [b*depth, (b+1)*depth) of the committed matrix, each block equal to
+    that vector committed alone."""
+    B, nv = 3, 10
+    p = WP.ProtocolParameters(security_level=32, pow_bits=0, initial_folding_factor=2, folding_factor=2,
+                              starting_log_inv_rate=1, batch_size=B)
+    cfg = WP.WhirConfig.new(1 << nv, p, 2)
+    p1 = WP.ProtocolParameters(security_level=32, pow_bits=0, initial_folding_factor=2, folding_factor=2,
+                               starting_log_inv_rate=1)
+    cfg1 = WP.WhirConfig.new(1 << nv, p1, 2)
+    vecs = [rand_vec(40 + b, 1 << nv) for b in range(B)]
+    wb = WP.commit_batch(WP.Transcript(b"a" * 64, b"b" * 32), cfg, vecs)
+    depth = cfg.initial.interleaving_depth
+    assert wb.matrix.shape == (cfg.initial.codeword_length, B * depth)
+    for b in range(B):
+        w1 = WP.commit(WP.Transcript(b"a" * 64, b"b" * 32), cfg1, vecs[b])
+        assert np.array_equal(wb.matrix[:, b * depth:(b + 1) * depth], w1.matrix)
+    # OOD values are point-major: value (i, b) = vector b at OOD point i
+    for i, x in enumerate(wb.ood_points):
+        for b in range(B):
+            assert np.array_equal(wb.ood_values[i * B + b], O.univariate_evaluate(1, 2, vecs[b], x))
+
+
+def test_batch_tampered_evaluation_is_rejected():
+    cfg, pid, sid, narg, hints, vecs, ws, evals, points = run_batch(10, 2, 2, 2, 2)
+    bad = [list(row) for row in evals]
+    bad[1][3] = WP.e_add(2, bad[1][3], WP.e_one(2))
+    with pytest.raises(WP.VerificationError):
+        WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 2, bad, points)
+    # swapping two commitments' vectors in the claim is caught too
+    swapped = [[row[2], row[3], row[0], row[1]] for row in evals]
+    with pytest.raises(WP.VerificationError):
+        WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 2, swapped, points)

@@ -111,6 +111,8 @@ SIGNATURES = {
     "whir_transcript_free": [vp],
     "whir_commit_dev": [vp, vp, vp, ctypes.POINTER(vp)],
     "whir_prove_dev": [vp, vp, vp, vp, u64p, sz, u64p, u64p],
+    "whir_commit_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)],
+    "whir_prove_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, u64p, sz, u64p, u64p],
     "whir_witness_free": [vp],
     "whir_config_memory": [vp, szp],
     "whir_executor_new": [ci, sz, ctypes.POINTER(vp)],

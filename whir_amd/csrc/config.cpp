@@ -87,7 +87,7 @@ WhirCfg WhirCfg::make(size_t size, const WhirParams& p) {
   // whir::Config::new, src/protocols/whir/config.rs:16-152
   WHIR_REQUIRE(size && (size & (size - 1)) == 0, "Only powers of two size are supported at the moment.");
   WHIR_REQUIRE(p.degree >= 1 && p.degree <= 3, "degree must be 1, 2 or 3");
-  WHIR_REQUIRE(p.batch_size == 1, "batch_size other than 1 is not supported by the GPU prover");
+  WHIR_REQUIRE(p.batch_size >= 1, "batch_size must be at least 1");
   WhirCfg c;
   c.D = p.degree;
   c.engine = p.hash_engine;

@@ -9,7 +9,8 @@
 //            (src/protocols/challenge_indices.rs:6-50)
 //   sumcheck sumcheck::Config::prove without masks (src/protocols/sumcheck.rs:69-162)
 //   pow      proof_of_work::Config::prove (src/protocols/proof_of_work.rs:68-166)
-// with one committed base-field vector and MultilinearExtension linear forms.
+// for base-field vectors committed batch_size at a time (one or more commitments
+// proven together) and MultilinearExtension linear forms.
 //
 // Every vector of the protocol (lifted vector, covector, codewords, Merkle trees)
 // lives in HBM for the whole proof; the host only sees the transcript values:
@@ -36,11 +37,13 @@ using namespace whir;
 namespace whir {
 
 struct DevWitness {
-  int Ds = 1;            // degree of the committed elements
-  size_t N = 0, M = 0;   // codeword length, number of messages (columns)
-  DevBuf cw;             // column-major codeword cw[j*N + r]
-  DevBuf nodes;          // Merkle nodes [padded leaves][layer]...[root]
-  std::vector<uint64_t> ood_points, ood_values;  // D words each
+  int Ds = 1;               // degree of the committed elements
+  size_t N = 0, M = 0;      // codeword length, number of messages (columns)
+  size_t num_vectors = 1;   // vectors in this commitment (M = num_vectors * interleaving depth)
+  DevBuf cw;                // column-major codeword cw[j*N + r]
+  DevBuf nodes;             // Merkle nodes [padded leaves][layer]...[root]
+  // D words each; ood_values is point-major (out_domain_samples x num_vectors)
+  std::vector<uint64_t> ood_points, ood_values;
 };
 
 }  // namespace whir
@@ -143,22 +146,26 @@ uint64_t coset_size(uint64_t masked_len, uint64_t N) {
   return c;
 }
 
-// irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (one vector,
-// no masks).  vec holds vector_size elements of Ds words on the device.
+// irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (no masks):
+// vecs[0..nvec) each hold vector_size elements of Ds words on the device; vector
+// b fills messages (columns) [b*depth, (b+1)*depth) of the committed matrix.
 template <int D>
-void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec, int Ds, DevWitness& w,
-                hipStream_t s) {
-  WHIR_REQUIRE(c.num_vectors == 1, "irs commit: one vector per commitment");
-  const size_t M = c.num_messages(), L = c.message_length(), N = c.codeword_length;
+void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* const* vecs, size_t nvec, int Ds,
+                DevWitness& w, hipStream_t s) {
+  WHIR_REQUIRE(nvec == c.num_vectors, "irs commit: the configuration commits to a different number of vectors");
+  const size_t depth = c.interleaving_depth, M = c.num_messages(), L = c.message_length(), N = c.codeword_length;
   uint64_t C = coset_size(L, N);
   w.Ds = Ds;
   w.N = N;
   w.M = M;
+  w.num_vectors = nvec;
   w.cw = DevBuf(M * N * Ds * 8, s);
-  RsArgs a{vec, L, L, nullptr, 0, w.cw.as<uint64_t>(), N, N / C, log2_exact(C), M};
   {
-    DevBuf scratch(rs_scratch_words(Ds, M, N, C) * 8, s);
-    rs_encode_cols(Ds, a, scratch.as<uint64_t>(), s);
+    DevBuf scratch(rs_scratch_words(Ds, depth, N, C) * 8, s);
+    for (size_t b = 0; b < nvec; b++) {
+      RsArgs a{vecs[b], L, L, nullptr, 0, w.cw.as<uint64_t>() + b * depth * N * Ds, N, N / C, log2_exact(C), depth};
+      rs_encode_cols(Ds, a, scratch.as<uint64_t>(), s);
+    }
   }
   size_t nn = merkle_num_nodes(N);
   w.nodes = DevBuf(32 * nn, s);
@@ -174,28 +181,41 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* vec,
   for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
   const size_t nood = c.out_domain_samples;
   if (nood) {
-    // All samples in shared passes over the vector, written to the mailbox.
-    WHIR_REQUIRE((nood + 3) / 4 * 4 * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain samples");
-    univariate_eval_many_dev(Ds, D, vec, c.vector_size, w.ood_points.data(), nood, mbox, s, true);
+    // All samples of a vector in shared passes over it, written to the mailbox
+    // (vector b at words [b*pad*D, (b+1)*pad*D)); sent point-major.
+    const size_t pad = (nood + 3) / 4 * 4;
+    WHIR_REQUIRE(nvec * pad * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain samples");
+    for (size_t b = 0; b < nvec; b++)
+      univariate_eval_many_dev(Ds, D, vecs[b], c.vector_size, w.ood_points.data(), nood, mbox + b * pad * D, s,
+                               true);
     WHIR_HIP(hipStreamSynchronize(s));
-    w.ood_values.assign(mbox, mbox + nood * D);
-    for (size_t i = 0; i < nood; i++) t.prover_field(D, &w.ood_values[i * D], 1);
+    w.ood_values.resize(nood * nvec * D);
+    for (size_t i = 0; i < nood; i++)
+      for (size_t b = 0; b < nvec; b++) {
+        std::copy(mbox + (b * pad + i) * D, mbox + (b * pad + i + 1) * D, &w.ood_values[(i * nvec + b) * D]);
+        t.prover_field(D, &w.ood_values[(i * nvec + b) * D], 1);
+      }
   }
 }
 
-// irs_commit::Config::open for one witness: hints the opened rows and their
-// Merkle paths; returns evaluation points (base field) and rows (Ds words).
-template <int D>
-void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<uint64_t>& pts,
-              std::vector<uint64_t>& rows, hipStream_t s) {
-  std::vector<size_t> idx = challenge_indices(t, c.codeword_length, c.in_domain_samples, c.deduplicate_in_domain);
-  const size_t N = w.N, M = w.M, k = idx.size();
+// in_domain_challenges (irs_commit.rs:486-500): indices and their evaluation points.
+void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::vector<uint64_t>& pts) {
+  idx = challenge_indices(t, c.codeword_length, c.in_domain_samples, c.deduplicate_in_domain);
+  const size_t N = c.codeword_length;
   uint64_t C = coset_size(c.masked_message_length(), N), Q = N / C;
   uint64_t g = gl::root_of_unity(N);
   pts.clear();
   for (size_t i : idx) pts.push_back(gl::pow(g, (i / C) + (i % C) * Q));  // transpose_permute
-  // merkle_tree::Config::open: sibling node indices in hint order (they depend on
-  // the indices alone, so rows and siblings are gathered in one launch).
+}
+
+// The per-witness part of irs_commit::Config::open (irs_commit.rs:410-428): hints
+// the rows at idx and their Merkle path (merkle_tree.rs:145-186); rows returned
+// (k x M elements of Ds words).
+void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& idx, std::vector<uint64_t>& rows,
+               hipStream_t s) {
+  const size_t N = w.N, M = w.M, k = idx.size();
+  // Sibling node indices in hint order (they depend on the indices alone, so rows
+  // and siblings are gathered in one launch).
   std::vector<size_t> cur(idx);
   std::sort(cur.begin(), cur.end());
   cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
@@ -243,6 +263,17 @@ void irs_open(Transcript& t, const IrsCfg& c, const DevWitness& w, std::vector<u
   }
   t.hint_field_vec(w.Ds, rows.data(), k * M);
   for (size_t i = 0; i < ns; i++) t.hint_hash(&hashes[32 * i]);
+}
+
+// irs_commit::Config::open (irs_commit.rs:385-432) over one or more witnesses:
+// one set of in-domain indices, then each witness's rows and Merkle path.
+// rows[w] holds witness w's opened rows.
+void irs_open(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness*>& ws, std::vector<uint64_t>& pts,
+              std::vector<std::vector<uint64_t>>& rows, hipStream_t s) {
+  std::vector<size_t> idx;
+  in_domain(t, c, idx, pts);
+  rows.resize(ws.size());
+  for (size_t w = 0; w < ws.size(); w++) open_rows(t, *ws[w], idx, rows[w], s);
 }
 
 // sumcheck::Config::prove (no masks), staggered fold + round polynomial.  The
@@ -305,22 +336,79 @@ std::vector<E<D>> eq_weights(const std::vector<E<D>>& z) {
 }
 
 template <int D>
-void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* vec, DevWitness& w, hipStream_t s) {
-  irs_commit<D>(t, cfg.initial, cfg.engine, vec, 1, w, s);
+void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, size_t nvec, DevWitness& w,
+                 hipStream_t s) {
+  irs_commit<D>(t, cfg.initial, cfg.engine, vecs, nvec, 1, w, s);
 }
 
-// whir::Config::prove, src/protocols/whir/prover.rs:50-316.
+// whir::Config::prove, src/protocols/whir/prover.rs:50-316: vecs[0..V) are the
+// committed base-field vectors, witnesses ws[0..W) hold them batch_size at a time
+// in order (V = W * batch_size); evals[(k*V + j)*D] is vector j at point k.
 template <int D>
-void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, const DevWitness& w0,
-                const uint64_t* points, size_t np, const uint64_t* evals, uint64_t* eval_point_out, hipStream_t s) {
+void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, size_t V,
+                const DevWitness* const* ws, size_t W, const uint64_t* points, size_t np, const uint64_t* evals,
+                uint64_t* eval_point_out, hipStream_t s) {
   const int eng = cfg.engine;
+  const size_t B = cfg.initial.num_vectors;
+  WHIR_REQUIRE(V > 0 && W > 0, "prove: no vectors");
+  WHIR_REQUIRE(V == W * B, "prove: the number of vectors must be the number of witnesses times batch_size");
+  for (size_t w = 0; w < W; w++)
+    WHIR_REQUIRE(ws[w] && ws[w]->num_vectors == B && ws[w]->N == cfg.initial.codeword_length &&
+                     ws[w]->M == cfg.initial.num_messages(),
+                 "prove: witness does not match the configuration");
   size_t n = cfg.initial.vector_size;
   size_t nv = 0;
   while (((size_t)1 << nv) < n) nv++;
-  // Random linear combination of the vectors (one vector: [1], no transcript).
-  std::vector<E<D>> vector_rlc = geometric_challenge<D>(t, 1);
+  uint64_t* mbox = result_mailbox(s);
+
+  // Complete the OOD matrix with its cross terms (prover.rs:98-125): the rows of
+  // witness w hold its own vectors; every other vector is evaluated at w's OOD
+  // points here and sent.
+  std::vector<uint64_t> opts;  // all OOD points, witness by witness
+  for (size_t w = 0; w < W; w++) opts.insert(opts.end(), ws[w]->ood_points.begin(), ws[w]->ood_points.end());
+  const size_t nood = opts.size() / D;
+  std::vector<uint64_t> cross;  // cross[(j*nood + o)*D]: vector j at OOD point o
+  if (W > 1 && nood) {
+    const size_t pad = (nood + 3) / 4 * 4;
+    WHIR_REQUIRE(V * pad * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain cross terms");
+    for (size_t j = 0; j < V; j++)
+      univariate_eval_many_dev(1, D, vecs[j], n, opts.data(), nood, mbox + j * pad * D, s, true);
+    WHIR_HIP(hipStreamSynchronize(s));
+    cross.resize(V * nood * D);
+    for (size_t j = 0; j < V; j++) std::copy(mbox + j * pad * D, mbox + (j * pad + nood) * D, &cross[j * nood * D]);
+  }
+  std::vector<E<D>> omat;  // nood x V
+  for (size_t w = 0, o = 0; w < W; w++) {
+    const size_t wn = ws[w]->ood_points.size() / D;
+    for (size_t i = 0; i < wn; i++, o++)
+      for (size_t j = 0; j < V; j++) {
+        if (j / B == w) {
+          omat.push_back(ld<D>(&ws[w]->ood_values[(i * B + j % B) * D]));
+        } else {
+          omat.push_back(ld<D>(&cross[(j * nood + o) * D]));
+          t.prover_field(D, omat.back().c, 1);
+        }
+      }
+  }
+
+  // Random linear combination of the vectors ([1] for one vector, no transcript).
+  std::vector<E<D>> vector_rlc = geometric_challenge<D>(t, V);
   DevBuf vec(n * D * 8, s), cov(n * D * 8, s);
-  const size_t nood = w0.ood_points.size() / D;
+  // The sumcheck's first operand: the committed vector itself while there is one
+  // (read from the base field without lifting it), else the combination.
+  const uint64_t* a0 = vecs[0];
+  int a0_Ds = 1;
+  if (V > 1) {
+    lift_dev(D, vecs[0], n, vec.as<uint64_t>(), s);
+    for (size_t j = 1; j < V; j++) mixed_scalar_mul_add_dev(D, vec.as<uint64_t>(), vector_rlc[j].c, vecs[j], n, s);
+    a0 = vec.as<uint64_t>();
+    a0_Ds = D;
+  }
+  auto rlc_dot = [&](const E<D>* row) {
+    E<D> acc = gl::ext_zero<D>();
+    for (size_t j = 0; j < V; j++) acc = gl::eadd<D>(acc, gl::emul<D>(vector_rlc[j], row[j]));
+    return acc;
+  };
   std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
   const bool has_constraints = !crlc.empty();
   {
@@ -328,60 +416,74 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
     std::vector<uint64_t> erlc, srlc;
     for (size_t k = 0; k < np; k++) push<D>(erlc, crlc[k]);
     for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
-    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), np, nv, srlc.data(),
-                 w0.ood_points.data(), nood, s);
+    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), np, nv, srlc.data(), opts.data(), nood, s);
   }
   E<D> the_sum = gl::ext_zero<D>();
-  for (size_t k = 0; k < np; k++)
-    the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[k], gl::emul<D>(vector_rlc[0], ld<D>(evals + k * D))));
-  for (size_t k = 0; k < nood; k++)
-    the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[np + k],
-                                               gl::emul<D>(vector_rlc[0], ld<D>(&w0.ood_values[k * D]))));
+  {
+    std::vector<E<D>> row(V);
+    for (size_t k = 0; k < np; k++) {
+      for (size_t j = 0; j < V; j++) row[j] = ld<D>(evals + (k * V + j) * D);
+      the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[k], rlc_dot(row.data())));
+    }
+    for (size_t o = 0; o < nood; o++) the_sum = gl::eadd<D>(the_sum, gl::emul<D>(crlc[np + o], rlc_dot(&omat[o * V])));
+  }
   std::vector<E<D>> fr;
   if (has_constraints) {
-    fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, dvector, 1, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
+    fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, a0, a0_Ds, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
                            the_sum, s);
   } else {
     for (size_t i = 0; i < cfg.initial_sumcheck.num_rounds; i++) fr.push_back(verifier_field<D>(t));
     pow_prove(t, eng, cfg.initial_skip_pow, s);
-    const uint64_t* src = dvector;
-    int src_Ds = 1;
+    const uint64_t* src = a0;
+    int src_Ds = a0_Ds;
     for (const auto& f : fr) {
       n = fold_into_dev(src_Ds, D, src, vec.as<uint64_t>(), n, f.c, s);
       src = vec.as<uint64_t>();
       src_Ds = D;
     }
-    if (fr.empty()) lift_dev(D, dvector, n, vec.as<uint64_t>(), s);
+    if (fr.empty() && a0 != vec.as<uint64_t>()) lift_dev(D, a0, n, vec.as<uint64_t>(), s);
     fill_zero(cov.ptr, n * D * 8, s);  // the covector is all zeros
   }
   std::vector<E<D>> eval_point(fr);
 
+  // In-domain value of opened row q: dot(tensor_product(poly_rlc, eq_weights(r)),
+  // row) over the rows of every opened witness side by side (irs_commit.rs:535-537).
+  auto row_values = [&](const std::vector<const DevWitness*>& pw, const std::vector<std::vector<uint64_t>>& rows,
+                        size_t k, const std::vector<E<D>>& poly_rlc, const std::vector<E<D>>& w_eq) {
+    std::vector<E<D>> out(k, gl::ext_zero<D>());
+    const size_t depth = w_eq.size();
+    for (size_t w = 0; w < pw.size(); w++) {
+      const size_t M = pw[w]->M;
+      const int Ds = pw[w]->Ds;
+      for (size_t col = 0; col < M; col++) {
+        const size_t g = w * M + col;  // column of the concatenated rows = vector * depth + chunk
+        E<D> wt = gl::emul<D>(poly_rlc[g / depth], w_eq[g % depth]);
+        for (size_t q = 0; q < k; q++) {
+          const uint64_t* e = &rows[w][(q * M + col) * Ds];
+          E<D> v = Ds == 1 ? gl::ext_base<D>(e[0]) : ld<D>(e);
+          out[q] = gl::eadd<D>(out[q], gl::emul<D>(wt, v));
+        }
+      }
+    }
+    return out;
+  };
+
   const IrsCfg* prev_cfg = &cfg.initial;
-  const DevWitness* prev_w = &w0;
+  std::vector<const DevWitness*> prev_ws(ws, ws + W);
   std::unique_ptr<DevWitness> held;
   for (const RoundCfg& rc : cfg.rounds) {
     auto nw = std::make_unique<DevWitness>();
-    irs_commit<D>(t, rc.irs, eng, vec.as<uint64_t>(), D, *nw, s);
+    const uint64_t* vp = vec.as<uint64_t>();
+    irs_commit<D>(t, rc.irs, eng, &vp, 1, D, *nw, s);
     pow_prove(t, eng, rc.pow, s);
-    std::vector<uint64_t> pts, rows;
-    irs_open<D>(t, *prev_cfg, *prev_w, pts, rows, s);
+    std::vector<uint64_t> pts;
+    std::vector<std::vector<uint64_t>> rows;
+    irs_open(t, *prev_cfg, prev_ws, pts, rows, s);
     // STIR constraints: OOD points of the new commitment, then the opened rows.
-    std::vector<E<D>> w_eq = eq_weights<D>(fr);  // tensor_product([1], eq_weights(r))
+    std::vector<E<D>> svals = row_values(prev_ws, rows, pts.size(), vector_rlc, eq_weights<D>(fr));
     std::vector<uint64_t> spoints(nw->ood_points);
-    std::vector<E<D>> svals;
-    for (size_t k = 0; k < nw->ood_values.size() / D; k++) svals.push_back(ld<D>(&nw->ood_values[k * D]));
-    const size_t M = prev_w->M;
-    const int Ds = prev_w->Ds;
-    for (size_t q = 0; q < pts.size(); q++) {
-      push<D>(spoints, gl::ext_base<D>(pts[q]));
-      E<D> acc = gl::ext_zero<D>();
-      for (size_t j = 0; j < M; j++) {
-        const uint64_t* e = &rows[(q * M + j) * Ds];
-        E<D> v = Ds == 1 ? gl::ext_base<D>(e[0]) : ld<D>(e);
-        acc = gl::eadd<D>(acc, gl::emul<D>(w_eq[j], v));
-      }
-      svals.push_back(acc);
-    }
+    for (uint64_t x : pts) push<D>(spoints, gl::ext_base<D>(x));
+    for (size_t k = nw->ood_values.size() / D; k-- > 0;) svals.insert(svals.begin(), ld<D>(&nw->ood_values[k * D]));
     std::vector<E<D>> srlc = geometric_challenge<D>(t, svals.size());
     if (!srlc.empty()) {
       std::vector<uint64_t> flat;
@@ -395,7 +497,8 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
     eval_point.insert(eval_point.end(), fr.begin(), fr.end());
     held = std::move(nw);
     prev_cfg = &rc.irs;
-    prev_w = held.get();
+    prev_ws.assign(1, held.get());
+    vector_rlc.assign(1, gl::ext_base<D>(1));
   }
   // Send the final vector directly.
   WHIR_REQUIRE(n == cfg.final_sumcheck.initial_size, "final vector length mismatch");
@@ -404,8 +507,9 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* dvector, cons
   for (size_t i = 0; i < n; i++) t.prover_field(D, &fin[i * D], 1);
   pow_prove(t, eng, cfg.final_pow, s);
   {
-    std::vector<uint64_t> pts, rows;
-    irs_open<D>(t, *prev_cfg, *prev_w, pts, rows, s);
+    std::vector<uint64_t> pts;
+    std::vector<std::vector<uint64_t>> rows;
+    irs_open(t, *prev_cfg, prev_ws, pts, rows, s);
   }
   fr = sumcheck_prove<D>(t, cfg.final_sumcheck, eng, vec.as<uint64_t>(), D, vec.as<uint64_t>(), cov.as<uint64_t>(),
                          n, the_sum, s);
@@ -516,29 +620,50 @@ int whir_transcript_proof(const whir_transcript* t, uint8_t* narg, size_t narg_c
 
 void whir_transcript_free(whir_transcript* t) { delete t; }
 
-int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, whir_witness** out) {
+int whir_commit_batch_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,
+                          whir_witness** out) {
   API_BEGIN
+  WHIR_REQUIRE(c && t && vectors && out, "commit: null argument");
+  for (size_t b = 0; b < nvec; b++) WHIR_REQUIRE(vectors[b], "commit: null vector");
   hipStream_t s = thread_stream();
   auto w = std::make_unique<whir_witness>();
   switch (c->cfg.D) {
-    case 1: commit_impl<1>(c->cfg, *t->t, vector, w->w, s); break;
-    case 2: commit_impl<2>(c->cfg, *t->t, vector, w->w, s); break;
-    case 3: commit_impl<3>(c->cfg, *t->t, vector, w->w, s); break;
+    case 1: commit_impl<1>(c->cfg, *t->t, vectors, nvec, w->w, s); break;
+    case 2: commit_impl<2>(c->cfg, *t->t, vectors, nvec, w->w, s); break;
+    case 3: commit_impl<3>(c->cfg, *t->t, vectors, nvec, w->w, s); break;
   }
   *out = w.release();
   API_END
 }
 
-int whir_prove_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, const whir_witness* w,
-                   const uint64_t* points, size_t np, const uint64_t* evals, uint64_t* eval_point) {
+int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, whir_witness** out) {
+  return whir_commit_batch_dev(c, t, &vector, 1, out);
+}
+
+int whir_prove_batch_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,
+                         const whir_witness* const* witnesses, size_t nw, const uint64_t* points, size_t np,
+                         const uint64_t* evals, uint64_t* eval_point) {
   API_BEGIN
+  WHIR_REQUIRE(c && t && vectors && witnesses, "prove: null argument");
+  WHIR_REQUIRE(np == 0 || (points && evals), "prove: null points or evaluations");
+  std::vector<const DevWitness*> ws(nw);
+  for (size_t i = 0; i < nw; i++) {
+    WHIR_REQUIRE(witnesses[i], "prove: null witness");
+    ws[i] = &witnesses[i]->w;
+  }
+  for (size_t j = 0; j < nvec; j++) WHIR_REQUIRE(vectors[j], "prove: null vector");
   hipStream_t s = thread_stream();
   switch (c->cfg.D) {
-    case 1: prove_impl<1>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
-    case 2: prove_impl<2>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
-    case 3: prove_impl<3>(c->cfg, *t->t, vector, w->w, points, np, evals, eval_point, s); break;
+    case 1: prove_impl<1>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
+    case 2: prove_impl<2>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
+    case 3: prove_impl<3>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
   }
   API_END
+}
+
+int whir_prove_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, const whir_witness* w,
+                   const uint64_t* points, size_t np, const uint64_t* evals, uint64_t* eval_point) {
+  return whir_prove_batch_dev(c, t, &vector, 1, &w, 1, points, np, evals, eval_point);
 }
 
 void whir_witness_free(whir_witness* w) { delete w; }

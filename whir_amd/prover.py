@@ -135,6 +135,35 @@ def prove(cfg: Config, t: Transcript, d_vector: DeviceBuffer, w: Witness, points
     return out[: nv * D]
 
 
+def commit_batch(cfg: Config, t: Transcript, d_vectors) -> Witness:
+    """whir::Config::commit of batch_size device vectors in one commitment."""
+    ptrs = (vp * max(len(d_vectors), 1))(*[d.ptr for d in d_vectors])
+    h = vp()
+    _call("whir_commit_batch_dev", cfg.h, t.h, ptrs, len(d_vectors), ctypes.byref(h))
+    return Witness(h.value)
+
+
+def prove_batch(cfg: Config, t: Transcript, d_vectors, witnesses, points, evaluations) -> np.ndarray:
+    """whir::Config::prove over several commitments (batch_size vectors each, in
+    commitment order).  points: (num_points, num_variables * degree) words;
+    evaluations: (num_points, num_vectors, degree), vector j at point k."""
+    D, nv = cfg.degree, cfg.num_variables
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.uint64)).reshape(-1)
+    ev = np.ascontiguousarray(np.asarray(evaluations, dtype=np.uint64)).reshape(-1)
+    npts = pts.size // (nv * D) if nv else len(evaluations)
+    assert ev.size == npts * len(d_vectors) * D, "evaluations: num_points x num_vectors values"
+    if pts.size == 0:
+        pts = np.zeros(1, dtype=np.uint64)
+    if ev.size == 0:
+        ev = np.zeros(1, dtype=np.uint64)
+    vptrs = (vp * max(len(d_vectors), 1))(*[d.ptr for d in d_vectors])
+    wptrs = (vp * max(len(witnesses), 1))(*[w.h for w in witnesses])
+    out = np.zeros(max(nv * D, 1), dtype=np.uint64)
+    _call("whir_prove_batch_dev", cfg.h, t.h, vptrs, len(d_vectors), wptrs, len(witnesses), _p64(pts), npts,
+          _p64(ev), _p64(out))
+    return out[: nv * D]
+
+
 def mle_evaluate(degree: int, d_vector: DeviceBuffer, num_variables: int, point) -> np.ndarray:
     pt = np.ascontiguousarray(np.asarray(point, dtype=np.uint64)).reshape(-1)
     if pt.size == 0:
