@@ -1,0 +1,90 @@
+// CPU check of the GPU formulations of Goldilocks arithmetic (whir_amd/csrc/gl.h
+// with GL_HOST_CARRY_TEST: the carry-chain add/sub, the 4-product 128-bit multiply,
+// the carry-chain reduction and mul_pow2) against unsigned __int128 arithmetic,
+// over every pair of edge values and a stream of pseudo-random pairs.
+// Prints "ok <count>" or the first mismatch; exit status 0 iff all agree.
+#define GL_HOST_CARRY_TEST 1
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "../whir_amd/csrc/gl.h"
+
+typedef unsigned __int128 u128;
+static const uint64_t P = 0xFFFFFFFF00000001ull;
+
+static uint64_t r_add(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a + b) % P); }
+static uint64_t r_sub(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a + P - b) % P); }
+static uint64_t r_mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % P); }
+static uint64_t r_pow2(uint64_t x, unsigned k) {
+  uint64_t r = x % P;
+  for (unsigned i = 0; i < k; i++) r = (uint64_t)(((u128)r * 2) % P);
+  return r;
+}
+
+static uint64_t state = 0x9E3779B97F4A7C15ull;
+static uint64_t rnd() {  // splitmix64
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
+
+static long checked = 0;
+static int check(const char* op, uint64_t a, uint64_t b, uint64_t got, uint64_t want) {
+  checked++;
+  if (got != want) {
+    printf("MISMATCH %s a=%016llx b=%016llx got=%016llx want=%016llx\n", op, (unsigned long long)a,
+           (unsigned long long)b, (unsigned long long)got, (unsigned long long)want);
+    return 1;
+  }
+  return 0;
+}
+
+static int pair(uint64_t a, uint64_t b) {
+  int bad = 0;
+  bad |= check("add", a, b, gl::add(a, b), r_add(a, b));
+  bad |= check("sub", a, b, gl::sub(a, b), r_sub(a, b));
+  bad |= check("mul", a, b, gl::mul(a, b), r_mul(a, b));
+  uint64_t lo, hi;
+  gl::mul128(a, b, lo, hi);
+  u128 prod = (u128)a * b;
+  bad |= check("mul128.lo", a, b, lo, (uint64_t)prod);
+  bad |= check("mul128.hi", a, b, hi, (uint64_t)(prod >> 64));
+  // reduce128 of arbitrary 128-bit values (not only products of canonical inputs)
+  bad |= check("reduce128", a, b, gl::reduce128(a, b), (uint64_t)((((u128)b << 64) | a) % P));
+  return bad;
+}
+
+int main(int argc, char** argv) {
+  long n_random = argc > 1 ? atol(argv[1]) : 2000000;
+  const uint64_t E = 0xFFFFFFFFull;
+  uint64_t edge[] = {0, 1, 2, 3, E - 1, E, E + 1, E + 2, 1ull << 32, (1ull << 32) + 1, 1ull << 63, (1ull << 63) - 1,
+                     P - 1, P - 2, P - E, P - E - 1, P - (1ull << 32), P / 2, P / 2 + 1, 0x00000000FFFFFFFEull,
+                     0xFFFFFFFE00000000ull, 0xFFFFFFFE00000001ull, 0xFFFFFFFEFFFFFFFFull,
+                     0x8000000000000000ull, 0x7FFFFFFF80000000ull, 0x00000001FFFFFFFFull};
+  const int ne = sizeof(edge) / sizeof(edge[0]);
+  int bad = 0;
+  for (int i = 0; i < ne && !bad; i++)
+    for (int j = 0; j < ne && !bad; j++) bad |= pair(edge[i], edge[j]);
+  // reduce128 also at the top of its input range
+  uint64_t top[] = {~0ull, ~0ull - 1, 0xFFFFFFFF00000000ull, 0x00000000FFFFFFFFull, P, P + 1};
+  for (int i = 0; i < 6 && !bad; i++)
+    for (int j = 0; j < 6 && !bad; j++)
+      bad |= check("reduce128", top[i], top[j], gl::reduce128(top[i], top[j]),
+                   (uint64_t)((((u128)top[j] << 64) | top[i]) % P));
+  for (int i = 0; i < ne && !bad; i++)
+    for (unsigned k = 0; k < 192 && !bad; k++)
+      bad |= check("mul_pow2", edge[i], k, gl::mul_pow2(canon(edge[i]), k), r_pow2(edge[i], k));
+  for (long t = 0; t < n_random && !bad; t++) {
+    uint64_t a = canon(rnd()), b = canon(rnd());
+    // mix in values near p and near the 32-bit boundaries
+    if ((t & 7) == 1) a = P - 1 - (rnd() & 0xFFFF);
+    if ((t & 7) == 2) b = (rnd() & 0xFFFFFFFFull);
+    if ((t & 7) == 3) a = (rnd() << 32) | 0xFFFFFFFFull, a = canon(a);
+    bad |= pair(a, b);
+    if ((t & 63) == 0) bad |= check("mul_pow2", a, t % 192, gl::mul_pow2(a, (unsigned)(t % 192)), r_pow2(a, (unsigned)(t % 192)));
+  }
+  if (!bad) printf("ok %ld\n", checked);
+  return bad;
+}

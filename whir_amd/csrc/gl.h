@@ -18,6 +18,14 @@
 #define GL_FN static inline
 #endif
 
+// GL_CARRY selects the formulations written for the GPU (branch-free selects,
+// explicit carry chains, a 128-bit product from 32x32->64 partial products).
+// Device code always uses them; a host build can select them with
+// GL_HOST_CARRY_TEST to check them against 128-bit integer arithmetic on the CPU.
+#if defined(__HIP_DEVICE_COMPILE__) || defined(GL_HOST_CARRY_TEST)
+#define GL_CARRY 1
+#endif
+
 namespace gl {
 
 constexpr uint64_t P = 0xFFFFFFFF00000001ull;
@@ -32,7 +40,7 @@ GL_FN uint64_t umulhi(uint64_t a, uint64_t b) {
 }
 
 GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(GL_CARRY)
   // Same steps with explicit carry chains, so every correction is a masked
   // subtract or add instead of a 64-bit compare and two selects.
   unsigned long c;
@@ -59,9 +67,10 @@ GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
 }
 
 GL_FN uint64_t add(uint64_t a, uint64_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(GL_CARRY)
   // Branch-free: one correction covers both cases.  If a + b wrapped, the sum is
   // s + 2^64 = s + EPS (mod p) and s + EPS < p; if s >= p, s - p = s + EPS mod 2^64.
+  // (A 32-bit carry-chain form of the same tests measured no faster on MI355X.)
   const uint64_t s = a + b, t = s + EPS;
   return ((s < a) | (t < s)) ? t : s;
 #else
@@ -71,7 +80,7 @@ GL_FN uint64_t add(uint64_t a, uint64_t b) {
 #endif
 }
 GL_FN uint64_t sub(uint64_t a, uint64_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(GL_CARRY)
   // The borrow of the 64-bit subtraction selects d + p; taking it from the
   // carry chain saves the separate compare.
   uint32_t c0, c1;
@@ -90,7 +99,7 @@ GL_FN uint64_t neg(uint64_t a) { return a ? P - a : 0; }
 // the previous one's spill as its addend (v_mad_u64_u32 on CDNA): no partial sum
 // overflows 64 bits, and the low product is not recomputed.
 GL_FN void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(GL_CARRY)
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   const uint64_t p00 = (uint64_t)a0 * b0;
   const uint64_t mid = (uint64_t)a0 * b1 + (p00 >> 32);           // <= 2^64 - 2^32
