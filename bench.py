@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="arrival rate; 0 = 70%% of measured capacity")
+    ap.add_argument("--ntt-autotune", type=int, default=1, choices=(0, 1),
+                    help="tune each NTT shape's schedule during the warmup steps (1, default) or use the "
+                         "built-in schedules (0)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -106,8 +109,13 @@ def main():
         GP.prove(cfg, t, dvec, w, flat_pts, flat_ev)
         return t
 
-    for _ in range(args.warmup):
+    W.ntt_autotune(bool(args.ntt_autotune))
+    for _ in range(max(args.warmup, 1 if args.ntt_autotune else 0)):
         step()
+    # Shapes tuned so far keep their schedules; later sections (other configurations,
+    # batching, serving) use the built-in schedules for new shapes, so no tuning
+    # time lands inside their measurements.
+    W.ntt_autotune(False)
     W.synchronize()
     t0 = time.perf_counter()
     last = None
@@ -196,6 +204,8 @@ def main():
         "batched": batched,
         "serving": serving,
         "kernels": kernels,
+        "ntt_schedules": [{k: e[k] for k in ("degree", "log_n", "log_c", "messages", "log_c1", "pass1", "pass2",
+                                             "tuned_ms", "default_ms")} for e in W.ntt_schedules()],
     }
     print(json.dumps(line))
 

@@ -102,6 +102,23 @@ int whir_rs_interleaved_encode_dev(int degree, const uint64_t *messages, size_t 
                                    size_t message_length, const uint64_t *masks, size_t mask_length,
                                    size_t codeword_length, uint64_t *out, int row_major);
 
+/* NTT schedule autotuner.  No counterpart in the reference (its NttEngine picks
+ * one fixed recursion, cooley_tukey.rs:240-360); for callers that encode the same
+ * shapes many times.  With tuning on, the first encode of each shape (degree,
+ * log2 N, log2 coset size, number of messages) times every candidate schedule
+ * (four-step split, per-pass LDS padding / radix-16 / tile size) on the caller's
+ * stream and keeps the fastest for the rest of the process; outputs do not depend
+ * on the schedule.  Pinned schedules are used whether or not tuning is on.
+ * Variants are 2-bit masks (bit 1: LDS padding, bit 0: radix 16); tiles are u64
+ * words of LDS (4096, or 8192 where the device allows it); log_c1 = 0 for shapes
+ * with log2 C <= 11 (one pass). */
+int whir_ntt_autotune(int enable);
+int whir_ntt_set_schedule(int degree, size_t log_n, size_t log_c, size_t num_messages, int log_c1, int variant1,
+                          int tile1, int variant2, int tile2);
+int whir_ntt_clear_schedules(void);
+/* JSON array of the schedule table.  Pass buf = NULL to query the length. */
+int whir_ntt_schedules(char *buf, size_t cap, size_t *len);
+
 /* ------------------------------------------------------------- HashEngine
  * Replaces HashEngine (src/hash/mod.rs:58-86) for the engines of the registry
  * (src/hash/mod.rs:28-36): Blake3 (src/hash/blake3_engine.rs:83-160), Sha2 / Sha3 /
@@ -311,6 +328,13 @@ void whir_executor_free(whir_executor *e);
 int whir_executor_submit(whir_executor *e, int kind, const whir_config *cfg, whir_transcript *t,
                          const uint64_t *vector, const uint64_t *points, size_t num_points,
                          const uint64_t *evaluations, uint64_t *job_id);
+/* A job proving num_vectors vectors (a multiple of batch_size) together: they are
+ * committed batch_size at a time in order (whir_commit_batch_dev), then proven with
+ * whir_prove_batch_dev; evaluations are num_points x num_vectors values.  The
+ * device vectors must stay valid until the job has been waited for. */
+int whir_executor_submit_batch(whir_executor *e, const whir_config *cfg, whir_transcript *t,
+                               const uint64_t *const *vectors, size_t num_vectors, const uint64_t *points,
+                               size_t num_points, const uint64_t *evaluations, uint64_t *job_id);
 /* Blocks until the job has finished and returns its status.  times (optional)
  * receives {submitted, started, finished} on the executor's clock (seconds);
  * eval_point (optional, num_variables * degree words) the folding randomness. */

@@ -206,3 +206,65 @@ def test_executor_failure_stays_with_its_job(mix):
         assert tb.proof() == expected[3][0]
     finally:
         ex.close()
+
+
+def batch_instance(seed, nv, B, NW, D=2, num_points=2):
+    rng = np.random.default_rng(seed)
+    vecs = [rng.integers(0, O.P, size=1 << nv, dtype=np.uint64) for _ in range(B * NW)]
+    pts = rng.integers(0, O.P, size=num_points * nv * D, dtype=np.uint64)
+    params = GP.ProtocolParameters(degree=D, security_level=32, pow_bits=0, initial_folding_factor=4,
+                                   folding_factor=4, starting_log_inv_rate=1, batch_size=B)
+    cfg = GP.Config(1 << nv, params).disable_pow()
+    dvs = [W.DeviceBuffer.from_array(v) for v in vecs]
+    evals = np.array([[GP.mle_evaluate(D, dv, nv, pts[k * nv * D:(k + 1) * nv * D]) for dv in dvs]
+                      for k in range(num_points)], dtype=np.uint64)
+    pid = hashlib.sha3_512(f"executor batch {seed} {nv} {B} {NW}".encode()).digest()
+    sid = hashlib.sha3_256(f"batch session {seed}".encode()).digest()
+    return dict(nv=nv, B=B, NW=NW, D=D, vecs=vecs, cfg=cfg, dvs=dvs, pts=pts, evals=evals, pid=pid, sid=sid)
+
+
+def plain_batch(bi):
+    t = GP.Transcript(bi["pid"], bi["sid"])
+    B = bi["B"]
+    ws = [GP.commit_batch(bi["cfg"], t, bi["dvs"][w * B:(w + 1) * B]) for w in range(bi["NW"])]
+    ep = GP.prove_batch(bi["cfg"], t, bi["dvs"], ws, bi["pts"], bi["evals"])
+    return t.proof(), ep
+
+
+def test_executor_batch_jobs_equal_plain(mix):
+    """Multi-vector jobs (several commitments of batch_size vectors in one proof)
+    interleaved with single-vector jobs: every proof equals the one made alone."""
+    insts, expected = mix
+    bis = [batch_instance(30 + k, nv, B, NW) for k, (nv, B, NW) in enumerate([(12, 2, 1), (10, 1, 3), (13, 2, 2),
+                                                                              (11, 3, 1)])]
+    bexp = [plain_batch(bi) for bi in bis]
+    ex = GP.Executor(workers=4)
+    try:
+        jobs = []
+        for k in range(4):
+            bi = bis[k]
+            tb = GP.Transcript(bi["pid"], bi["sid"])
+            jobs.append(("b", k, ex.submit_batch(bi["cfg"], tb, bi["dvs"], bi["pts"], bi["evals"]), tb))
+            inst = insts[k]
+            ts = GP.Transcript(inst["pid"], inst["sid"])
+            jobs.append(("s", k, ex.submit(inst["cfg"], ts, inst["dv"], inst["pts"], inst["evals"]), ts))
+        for kind, k, jid, t in jobs:
+            _, ep = ex.wait(jid)
+            if kind == "b":
+                assert t.proof() == bexp[k][0], f"batch job {k} differs"
+                assert np.array_equal(ep, bexp[k][1])
+            else:
+                assert t.proof() == expected[k][0]
+    finally:
+        ex.close()
+
+
+def test_executor_batch_job_rejects_a_partial_batch():
+    bi = batch_instance(40, 10, 2, 1)
+    ex = GP.Executor(workers=1)
+    try:
+        with pytest.raises(W.WhirError, match="multiple of batch_size"):
+            ex.submit_batch(bi["cfg"], GP.Transcript(bi["pid"], bi["sid"]), bi["dvs"][:1],
+                            bi["pts"], bi["evals"][:, :1])
+    finally:
+        ex.close()

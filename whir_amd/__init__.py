@@ -111,6 +111,10 @@ SIGNATURES = {
     "whir_transcript_free": [vp],
     "whir_commit_dev": [vp, vp, vp, ctypes.POINTER(vp)],
     "whir_prove_dev": [vp, vp, vp, vp, u64p, sz, u64p, u64p],
+    "whir_ntt_autotune": [ci],
+    "whir_ntt_set_schedule": [ci, sz, sz, sz, ci, ci, ci, ci, ci],
+    "whir_ntt_clear_schedules": [],
+    "whir_ntt_schedules": [ctypes.c_char_p, sz, szp],
     "whir_commit_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)],
     "whir_prove_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, u64p, sz, u64p, u64p],
     "whir_witness_free": [vp],
@@ -118,6 +122,8 @@ SIGNATURES = {
     "whir_executor_new": [ci, sz, ctypes.POINTER(vp)],
     "whir_executor_free": [vp],
     "whir_executor_submit": [vp, ci, vp, vp, vp, u64p, sz, u64p, ctypes.POINTER(ctypes.c_uint64)],
+    "whir_executor_submit_batch": [vp, vp, vp, ctypes.POINTER(vp), sz, u64p, sz, u64p,
+                                   ctypes.POINTER(ctypes.c_uint64)],
     "whir_executor_wait": [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), u64p],
     "whir_executor_clock": [vp, ctypes.POINTER(ctypes.c_double)],
     "whir_mle_evaluate_dev": [ci, vp, sz, u64p, u64p],
@@ -211,6 +217,31 @@ def interleaved_rs_encode(messages, masks, codeword_length: int, degree: int = 1
     out = np.zeros(max(codeword_length * M * degree, 1), dtype=np.uint64)
     _call("whir_rs_interleaved_encode", degree, ptrs, M, L, _p64(mk), ml, codeword_length, _p64(out))
     return out[: codeword_length * M * degree].reshape(codeword_length, M * degree)
+
+
+def ntt_autotune(enable: bool = True) -> None:
+    """Tune the schedule of each new encode shape on its first use (see
+    whir_ntt_autotune); outputs do not depend on the schedule."""
+    _call("whir_ntt_autotune", int(bool(enable)))
+
+
+def ntt_set_schedule(degree: int, log_n: int, log_c: int, num_messages: int, log_c1: int, variant1: int,
+                     tile1: int, variant2: int, tile2: int) -> None:
+    _call("whir_ntt_set_schedule", degree, log_n, log_c, num_messages, log_c1, variant1, tile1, variant2, tile2)
+
+
+def ntt_clear_schedules() -> None:
+    _call("whir_ntt_clear_schedules")
+
+
+def ntt_schedules() -> list:
+    """The schedule table: one dict per shape (tuned or pinned)."""
+    import json
+    n = sz()
+    _call("whir_ntt_schedules", None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _call("whir_ntt_schedules", buf, n.value + 1, ctypes.byref(n))
+    return json.loads(buf.value.decode())
 
 
 # ------------------------------------------------------------------ HashEngine

@@ -237,6 +237,39 @@ int whir_rs_interleaved_encode_dev(int D, const uint64_t* msgs, size_t M, size_t
   API_END
 }
 
+int whir_ntt_autotune(int enable) {
+  API_BEGIN
+  ntt_autotune(enable != 0);
+  API_END
+}
+
+int whir_ntt_set_schedule(int D, size_t log_n, size_t log_c, size_t messages, int log_c1, int variant1, int tile1,
+                          int variant2, int tile2) {
+  API_BEGIN
+  WHIR_REQUIRE(log_n <= 40 && log_c <= 40, "ntt schedule: sizes out of range");
+  ntt_set_schedule(D, (int)log_n, (int)log_c, messages, log_c1, variant1, tile1, variant2, tile2);
+  API_END
+}
+
+int whir_ntt_clear_schedules(void) {
+  API_BEGIN
+  ntt_clear_schedules();
+  API_END
+}
+
+int whir_ntt_schedules(char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  WHIR_REQUIRE(len, "ntt_schedules: null length");
+  std::string js = ntt_schedules_json();
+  *len = js.size();
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, js.size());
+    memcpy(buf, js.data(), n);
+    buf[n] = 0;
+  }
+  API_END
+}
+
 // ------------------------------------------------------------------ hashes
 int whir_hash_engine_id(int engine, uint8_t out[32]) {
   API_BEGIN

@@ -41,7 +41,9 @@ namespace {
 // Peak device bytes of one commit + prove: the covector and folded vector, the
 // initial codeword (plus its NTT scratch) and Merkle nodes, and the largest
 // round witness alive next to it.
-size_t job_memory(const WhirCfg& c) {
+// With several commitments (multi-vector jobs) every initial witness stays alive
+// until the first round has opened it.
+size_t job_memory(const WhirCfg& c, size_t commitments = 1) {
   auto wit = [](const IrsCfg& irs, int Ds) {
     const size_t N = irs.codeword_length, M = irs.num_messages();
     return 2 * M * N * (size_t)Ds * 8 + 64 * N;
@@ -49,7 +51,7 @@ size_t job_memory(const WhirCfg& c) {
   const size_t n = c.initial.vector_size;
   size_t mx = 0;
   for (const RoundCfg& r : c.rounds) mx = std::max(mx, wit(r.irs, c.D));
-  return 2 * n * (size_t)c.D * 8 + wit(c.initial, 1) + mx;
+  return 2 * n * (size_t)c.D * 8 + commitments * wit(c.initial, 1) + mx;
 }
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -58,10 +60,11 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 struct Job {
   uint64_t id = 0;
-  int kind = 0;  // 0: commit + prove, 1: commit only
+  int kind = 0;  // 0: commit + prove, 1: commit only, 2: batch_size-vector commitments + one proof
   const whir_config* cfg = nullptr;
   whir_transcript* t = nullptr;
   const uint64_t* vector = nullptr;
+  std::vector<const uint64_t*> vectors;  // kind 2
   std::vector<uint64_t> points, evals;
   size_t np = 0;
   size_t mem = 0;
@@ -105,19 +108,38 @@ struct whir_executor {
     int st = 0;
     std::string err;
     std::vector<uint64_t> ep;
-    whir_witness* w = nullptr;
-    st = whir_commit_dev(j->cfg, j->t, j->vector, &w);
-    if (st == 0 && j->kind == 0) {
-      const WhirCfg& c = j->cfg->cfg;
-      size_t rounds = c.initial_sumcheck.num_rounds + c.final_sumcheck.num_rounds;
-      for (const RoundCfg& r : c.rounds) rounds += r.sumcheck.num_rounds;
-      ep.assign(rounds * (size_t)c.D, 0);  // one folding challenge per variable
-      st = whir_prove_dev(j->cfg, j->t, j->vector, w, j->points.data(), j->np, j->evals.data(), ep.data());
-    } else if (st == 0) {
-      st = hipStreamSynchronize(thread_stream()) == hipSuccess ? 0 : 1;
+    const WhirCfg& c = j->cfg->cfg;
+    size_t rounds = c.initial_sumcheck.num_rounds + c.final_sumcheck.num_rounds;
+    for (const RoundCfg& r : c.rounds) rounds += r.sumcheck.num_rounds;
+    if (j->kind == 2) {
+      // Commit each group of batch_size vectors in order, then prove them together.
+      const size_t B = c.initial.num_vectors, nw = j->vectors.size() / B;
+      std::vector<whir_witness*> ws;
+      for (size_t g = 0; g < nw && st == 0; g++) {
+        whir_witness* w = nullptr;
+        st = whir_commit_batch_dev(j->cfg, j->t, j->vectors.data() + g * B, B, &w);
+        if (w) ws.push_back(w);
+      }
+      if (st == 0) {
+        ep.assign(rounds * (size_t)c.D, 0);
+        std::vector<const whir_witness*> cws(ws.begin(), ws.end());
+        st = whir_prove_batch_dev(j->cfg, j->t, j->vectors.data(), j->vectors.size(), cws.data(), cws.size(),
+                                  j->points.data(), j->np, j->evals.data(), ep.data());
+      }
+      if (st) err = whir_last_error();
+      for (whir_witness* w : ws) whir_witness_free(w);
+    } else {
+      whir_witness* w = nullptr;
+      st = whir_commit_dev(j->cfg, j->t, j->vector, &w);
+      if (st == 0 && j->kind == 0) {
+        ep.assign(rounds * (size_t)c.D, 0);  // one folding challenge per variable
+        st = whir_prove_dev(j->cfg, j->t, j->vector, w, j->points.data(), j->np, j->evals.data(), ep.data());
+      } else if (st == 0) {
+        st = hipStreamSynchronize(thread_stream()) == hipSuccess ? 0 : 1;
+      }
+      if (st) err = whir_last_error();
+      if (w) whir_witness_free(w);
     }
-    if (st) err = whir_last_error();
-    if (w) whir_witness_free(w);
     std::lock_guard<std::mutex> g(mu);
     j->status = st;
     j->error = err;
@@ -227,6 +249,41 @@ int whir_executor_submit(whir_executor* e, int kind, const whir_config* cfg, whi
   {
     std::lock_guard<std::mutex> g(e->mu);
     WHIR_REQUIRE(!e->stop, "executor_submit: executor is shutting down");
+    j->id = e->next_id++;
+    j->t_submit = now_s(e->t0);
+    *job_id = j->id;
+    e->queue.push_back(j.get());
+    e->jobs.emplace(j->id, std::move(j));
+  }
+  e->cv_work.notify_one();
+  EXEC_API_END
+}
+
+int whir_executor_submit_batch(whir_executor* e, const whir_config* cfg, whir_transcript* t,
+                               const uint64_t* const* vectors, size_t num_vectors, const uint64_t* points,
+                               size_t num_points, const uint64_t* evaluations, uint64_t* job_id) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(e && cfg && t && vectors && job_id, "executor_submit_batch: null argument");
+  const size_t B = cfg->cfg.initial.num_vectors;
+  WHIR_REQUIRE(num_vectors > 0 && num_vectors % B == 0,
+               "executor_submit_batch: the number of vectors must be a multiple of batch_size");
+  for (size_t i = 0; i < num_vectors; i++) WHIR_REQUIRE(vectors[i], "executor_submit_batch: null vector");
+  if (num_points) WHIR_REQUIRE(points && evaluations, "executor_submit_batch: null points or evaluations");
+  auto j = std::make_unique<Job>();
+  j->kind = 2;
+  j->cfg = cfg;
+  j->t = t;
+  j->vectors.assign(vectors, vectors + num_vectors);
+  j->np = num_points;
+  size_t nv = 0;
+  while (((size_t)1 << nv) < cfg->cfg.initial.vector_size) nv++;
+  const size_t D = (size_t)cfg->cfg.D;
+  j->points.assign(points, points + num_points * nv * D);
+  j->evals.assign(evaluations, evaluations + num_points * num_vectors * D);
+  j->mem = job_memory(cfg->cfg, num_vectors / B);
+  {
+    std::lock_guard<std::mutex> g(e->mu);
+    WHIR_REQUIRE(!e->stop, "executor_submit_batch: executor is shutting down");
     j->id = e->next_id++;
     j->t_submit = now_s(e->t0);
     *job_id = j->id;

@@ -26,8 +26,11 @@
 #include "gl.h"
 #include "ntt.h"
 
+#include <algorithm>
 #include <cstdio>
+#include <string>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -75,7 +78,7 @@ namespace {
 std::mutex g_tw_mu;
 std::map<std::pair<int, int>, TwTable> g_tw;                                   // (device, log n)
 std::map<std::pair<int, int>, const uint64_t*> g_flat;                         // (device, log S)
-std::map<std::tuple<int, int, int, int>, const uint64_t*> g_shape;             // (device, kind, log N, log C)
+std::map<std::tuple<int, int, int, int, int>, const uint64_t*> g_shape;  // (device, kind, log N, log C, log C1)
 
 const uint64_t* upload_persistent(const std::vector<uint64_t>& v) {
   void* p = nullptr;
@@ -160,7 +163,7 @@ static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipS
   int dev = current_device();
   {
     std::lock_guard<std::mutex> g(g_tw_mu);
-    auto it = g_shape.find({dev, kind, logN, logC});
+    auto it = g_shape.find({dev, kind, logN, logC, logC1});
     if (it != g_shape.end()) return it->second;
   }
   TwTable twN = twiddles(logN);
@@ -173,7 +176,7 @@ static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipS
   WHIR_HIP(hipGetLastError());
   WHIR_HIP(hipStreamSynchronize(st));
   std::lock_guard<std::mutex> g(g_tw_mu);
-  auto res = g_shape.emplace(std::make_tuple(dev, kind, logN, logC), (const uint64_t*)p);
+  auto res = g_shape.emplace(std::make_tuple(dev, kind, logN, logC, logC1), (const uint64_t*)p);
   if (!res.second) (void)hipFree(p);  // another thread built it first
   return res.first->second;
 }
@@ -602,11 +605,28 @@ static const NttVariant& ntt_variant() {
 }
 static int variant_bits(int D) { return (((ntt_variant().pad >> D) & 1) << 1) | ((ntt_variant().r16 >> D) & 1); }
 
+// One encode's schedule: the four-step split and, per pass, the in-LDS variant
+// (bit 1: padding, bit 0: radix 16) and the LDS tile size in u64 words (which sets
+// how many columns / rows / sequences a block transforms).
+struct NttSched {
+  int logC1 = 0;
+  int v1 = 0, v2 = 0;
+  int tw1 = 4096, tw2 = 4096;
+};
+
+static NttSched default_sched(int D, int logC) {
+  NttSched s;
+  s.logC1 = logC <= MAX_PASS_LOG ? 0 : (logC + 1) / 2;
+  s.v1 = s.v2 = variant_bits(D);
+  s.tw1 = s.tw2 = tile_words();
+  return s;
+}
+
 template <int D, int EPT, bool PAD, bool R16>
-static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
+static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, int tw, hipStream_t st) {
   const int logC = a.logC, logC2 = logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
-  int logb = std::min(logC1, floor_log2(std::max(1, tile_words() / (C2 * D))));
+  int logb = std::min(logC1, floor_log2(std::max(1, tw / (C2 * D))));
   if (EPT > 0) logb = std::min(logb, std::max(0, floor_log2(EPT * NTT_THREADS) - logC2));
   const int tile = C2 << logb;
   WHIR_REQUIRE(EPT == 0 || tile <= EPT * NTT_THREADS, "ntt pass 1: tile larger than the register budget");
@@ -625,12 +645,13 @@ static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN,
 }
 
 template <int D, int EPT>
-static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, hipStream_t st) {
-  switch (variant_bits(D)) {
-    case 0: pass1_t<D, EPT, false, false>(a, post, scratch, logN, logC1, st); break;
-    case 1: pass1_t<D, EPT, false, true>(a, post, scratch, logN, logC1, st); break;
-    case 2: pass1_t<D, EPT, true, false>(a, post, scratch, logN, logC1, st); break;
-    default: pass1_t<D, EPT, true, true>(a, post, scratch, logN, logC1, st); break;
+static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, int logC1, int v, int tw,
+                  hipStream_t st) {
+  switch (v) {
+    case 0: pass1_t<D, EPT, false, false>(a, post, scratch, logN, logC1, tw, st); break;
+    case 1: pass1_t<D, EPT, false, true>(a, post, scratch, logN, logC1, tw, st); break;
+    case 2: pass1_t<D, EPT, true, false>(a, post, scratch, logN, logC1, tw, st); break;
+    default: pass1_t<D, EPT, true, true>(a, post, scratch, logN, logC1, tw, st); break;
   }
 }
 
@@ -645,11 +666,26 @@ static void single_t(RsArgs a, const uint64_t* pre, int G, size_t nseq, hipStrea
   WHIR_HIP(hipGetLastError());
 }
 
+template <int D>
+static void single(RsArgs a, int v, int tw, hipStream_t st) {
+  const int logN = log2_exact(a.N);
+  const uint64_t* pre = shape_table(0, logN, a.logC, 0, st);
+  const int C = 1 << a.logC;
+  const int G = 1 << std::max(0, floor_log2(std::max(1, tw / (C * D))));
+  const size_t nseq = (size_t)a.M * a.Q;
+  switch (v) {
+    case 0: single_t<D, false, false>(a, pre, G, nseq, st); break;
+    case 1: single_t<D, false, true>(a, pre, G, nseq, st); break;
+    case 2: single_t<D, true, false>(a, pre, G, nseq, st); break;
+    default: single_t<D, true, true>(a, pre, G, nseq, st); break;
+  }
+}
+
 template <int D, bool PAD, bool R16>
-static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, hipStream_t st) {
+static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, int tw, hipStream_t st) {
   const int logC2 = a.logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
-  int logb = std::min(logC2, floor_log2(std::max(1, tile_words() / (C1 * D))));
+  int logb = std::min(logC2, floor_log2(std::max(1, tw / (C1 * D))));
   size_t smem = (padded<PAD>((size_t)C1 << logb) * D + C1) * 8;
   dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
   allow_lds(k_ntt_pass2<D, PAD, R16>, smem);
@@ -659,33 +695,207 @@ static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, hipStream_t st) {
 }
 
 template <int D>
-static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
-  const int logN = log2_exact(a.N);
-  const int logC = a.logC;
-  const size_t nseq = (size_t)a.M * a.Q;
-  const int TW = tile_words();
+static void pass2(RsArgs a, uint64_t* scratch, int logC1, int v, int tw, hipStream_t st) {
+  switch (v) {
+    case 0: pass2_t<D, false, false>(a, scratch, logC1, tw, st); break;
+    case 1: pass2_t<D, false, true>(a, scratch, logC1, tw, st); break;
+    case 2: pass2_t<D, true, false>(a, scratch, logC1, tw, st); break;
+    default: pass2_t<D, true, true>(a, scratch, logC1, tw, st); break;
+  }
+}
+
+// ------------------------------------------------------------------ autotuner
+// With tuning on (whir_ntt_autotune(1) or WHIR_NTT_AUTOTUNE=1), the first encode of
+// each shape (degree, log N, log C, messages) times every candidate schedule on the
+// caller's stream, with the caller's data, and keeps the fastest:
+//   the split log C1 in {ceil(logC/2), floor(logC/2)} (two-pass shapes),
+//   per pass, the variant (padding x radix 16) and the tile size (4096 or 8192
+//   words, where the device's LDS allows it),
+// each pass tuned on its own (pass 1, then pass 2 on pass 1's output), each
+// candidate timed as the fastest of three runs after a warm one.  A candidate
+// displaces the built-in choice only when it is 3% faster.  Every candidate computes the same transform, so the tuned
+// encode's output is the untuned one; only its time differs.  Shapes tuned are
+// kept for the life of the process (whir_ntt_schedules reports them).
+struct SchedKey {
+  int D, logN, logC;
+  uint64_t M;
+  bool operator<(const SchedKey& o) const {
+    return std::tie(D, logN, logC, M) < std::tie(o.D, o.logN, o.logC, o.M);
+  }
+};
+struct SchedEntry {
+  NttSched s;
+  double ms1 = 0, ms2 = 0, ms_default = 0;
+  int candidates = 0;
+};
+static std::mutex g_sched_mu;
+static std::map<SchedKey, SchedEntry> g_sched;
+static int g_autotune = -1;  // -1: not read yet
+
+static bool autotune_on() {
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  if (g_autotune < 0) {
+    const char* e = getenv("WHIR_NTT_AUTOTUNE");
+    g_autotune = (e && atoi(e) > 0) ? 1 : 0;
+  }
+  return g_autotune == 1;
+}
+
+static std::vector<int> tile_candidates() {
+  std::vector<int> out{4096};
+  int dev = 0, lds = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      (size_t)(8192 + 8192 / 32 + 2048) * 8 <= (size_t)lds)
+    out.push_back(8192);
+  return out;
+}
+
+// One warm run, then the fastest of three timed runs, of fn on stream st.
+template <class F>
+static double time_on(hipStream_t st, F fn) {
+  hipEvent_t e0, e1;
+  WHIR_HIP(hipEventCreate(&e0));
+  WHIR_HIP(hipEventCreate(&e1));
+  fn();
+  float best = 1e30f;
+  for (int r = 0; r < 3; r++) {
+    WHIR_HIP(hipEventRecord(e0, st));
+    fn();
+    WHIR_HIP(hipEventRecord(e1, st));
+    WHIR_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    WHIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    best = std::min(best, ms);
+  }
+  WHIR_HIP(hipEventDestroy(e0));
+  WHIR_HIP(hipEventDestroy(e1));
+  return best;
+}
+
+// A candidate replaces the built-in choice only when it is at least this much
+// faster, so timing noise does not move a schedule away from the measured default.
+static constexpr double kTuneMargin = 0.97;
+
+template <int D>
+static SchedEntry tune(RsArgs a, uint64_t* scratch, hipStream_t st) {
+  const int logN = log2_exact(a.N), logC = a.logC;
+  const NttSched def = default_sched(D, logC);
+  const std::vector<int> tws = tile_candidates();
+  SchedEntry best;
+  best.s = def;
   if (logC <= MAX_PASS_LOG) {
-    const uint64_t* pre = shape_table(0, logN, logC, 0, st);
-    int C = 1 << logC;
-    int G = 1 << std::max(0, floor_log2(std::max(1, TW / (C * D))));
-    switch (variant_bits(D)) {
-      case 0: single_t<D, false, false>(a, pre, G, nseq, st); break;
-      case 1: single_t<D, false, true>(a, pre, G, nseq, st); break;
-      case 2: single_t<D, true, false>(a, pre, G, nseq, st); break;
-      default: single_t<D, true, true>(a, pre, G, nseq, st); break;
+    best.ms_default = time_on(st, [&] { single<D>(a, def.v1, def.tw1, st); });
+    best.ms1 = best.ms_default;
+    for (int v = 0; v < 4; v++)
+      for (int tw : tws) {
+        if (v == def.v1 && tw == def.tw1) continue;
+        double ms = time_on(st, [&] { single<D>(a, v, tw, st); });
+        best.candidates++;
+        if (ms < kTuneMargin * best.ms1) {
+          best.ms1 = ms;
+          best.s.v1 = v;
+          best.s.tw1 = tw;
+        }
+      }
+    single<D>(a, best.s.v1, best.s.tw1, st);  // leave the tuned schedule's output behind
+    return best;
+  }
+  constexpr int EPT = pass1_ept<D>();
+  double best_total = 1e30;
+  std::vector<int> splits{def.logC1};
+  if (logC % 2) splits.push_back(logC / 2);
+  for (int logC1 : splits) {
+    const int logC2 = logC - logC1;
+    if (logC1 > MAX_PASS_LOG || logC2 > MAX_PASS_LOG) continue;
+    if (EPT > 0 && (1 << logC2) > EPT * NTT_THREADS) continue;  // pass-1 tile must fit the registers
+    const uint64_t* post = shape_table(1, logN, logC, logC1, st);
+    // Each pass starts from the built-in choice and moves only for a clear win.
+    NttSched s = def;
+    s.logC1 = logC1;
+    double ms1 = time_on(st, [&] { pass1<D, EPT>(a, post, scratch, logN, logC1, def.v1, def.tw1, st); });
+    for (int v = 0; v < 4; v++)
+      for (int tw : tws) {
+        if (v == def.v1 && tw == def.tw1) continue;
+        double ms = time_on(st, [&] { pass1<D, EPT>(a, post, scratch, logN, logC1, v, tw, st); });
+        best.candidates++;
+        if (ms < kTuneMargin * ms1) {
+          ms1 = ms;
+          s.v1 = v;
+          s.tw1 = tw;
+        }
+      }
+    pass1<D, EPT>(a, post, scratch, logN, logC1, s.v1, s.tw1, st);
+    double ms2 = time_on(st, [&] { pass2<D>(a, scratch, logC1, def.v2, def.tw2, st); });
+    for (int v = 0; v < 4; v++)
+      for (int tw : tws) {
+        if (v == def.v2 && tw == def.tw2) continue;
+        double ms = time_on(st, [&] { pass2<D>(a, scratch, logC1, v, tw, st); });
+        best.candidates++;
+        if (ms < kTuneMargin * ms2) {
+          ms2 = ms;
+          s.v2 = v;
+          s.tw2 = tw;
+        }
+      }
+    if (logC1 == def.logC1)
+      best.ms_default = time_on(st, [&] {
+        pass1<D, EPT>(a, post, scratch, logN, logC1, def.v1, def.tw1, st);
+        pass2<D>(a, scratch, logC1, def.v2, def.tw2, st);
+      });
+    // The other split must beat the default split's best clearly too.
+    if (best_total > 1e29 || ms1 + ms2 < kTuneMargin * best_total) {
+      best_total = ms1 + ms2;
+      best.s = s;
+      best.ms1 = ms1;
+      best.ms2 = ms2;
     }
+  }
+  // Leave the tuned schedule's output behind.
+  const uint64_t* post = shape_table(1, logN, logC, best.s.logC1, st);
+  pass1<D, EPT>(a, post, scratch, logN, best.s.logC1, best.s.v1, best.s.tw1, st);
+  pass2<D>(a, scratch, best.s.logC1, best.s.v2, best.s.tw2, st);
+  return best;
+}
+
+template <int D>
+static void run_sched(RsArgs a, uint64_t* scratch, const NttSched& s, hipStream_t st) {
+  if (a.logC <= MAX_PASS_LOG) {
+    single<D>(a, s.v1, s.tw1, st);
     return;
   }
-  WHIR_REQUIRE(logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
-  const int logC1 = (logC + 1) / 2;
-  const uint64_t* post = shape_table(1, logN, logC, logC1, st);
-  pass1<D, pass1_ept<D>()>(a, post, scratch, logN, logC1, st);
-  switch (variant_bits(D)) {
-    case 0: pass2_t<D, false, false>(a, scratch, logC1, st); break;
-    case 1: pass2_t<D, false, true>(a, scratch, logC1, st); break;
-    case 2: pass2_t<D, true, false>(a, scratch, logC1, st); break;
-    default: pass2_t<D, true, true>(a, scratch, logC1, st); break;
+  WHIR_REQUIRE(a.logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
+  const int logN = log2_exact(a.N);
+  const uint64_t* post = shape_table(1, logN, a.logC, s.logC1, st);
+  pass1<D, pass1_ept<D>()>(a, post, scratch, logN, s.logC1, s.v1, s.tw1, st);
+  pass2<D>(a, scratch, s.logC1, s.v2, s.tw2, st);
+}
+
+template <int D>
+static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
+  WHIR_REQUIRE(a.logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
+  const SchedKey key{D, log2_exact(a.N), a.logC, a.M};
+  NttSched s;
+  bool found = false;
+  {
+    std::lock_guard<std::mutex> g(g_sched_mu);
+    auto it = g_sched.find(key);
+    if (it != g_sched.end()) {
+      s = it->second.s;
+      found = true;
+    }
   }
+  if (found) {
+    run_sched<D>(a, scratch, s, st);
+    return;
+  }
+  if (!autotune_on()) {
+    run_sched<D>(a, scratch, default_sched(D, a.logC), st);
+    return;
+  }
+  SchedEntry e = tune<D>(a, scratch, st);  // also leaves the output behind
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  g_sched.emplace(key, e);
 }
 
 size_t rs_scratch_words(int D, uint64_t M, uint64_t N, uint64_t C) {
@@ -703,6 +913,56 @@ void rs_encode_cols(int D, RsArgs a, uint64_t* scratch, hipStream_t st) {
     case 3: launch<3>(a, scratch, st); break;
     default: throw Error("rs_encode: extension degree must be 1, 2 or 3");
   }
+}
+
+void ntt_autotune(bool on) {
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  g_autotune = on ? 1 : 0;
+}
+
+void ntt_set_schedule(int D, int logN, int logC, uint64_t M, int logC1, int v1, int tw1, int v2, int tw2) {
+  WHIR_REQUIRE(D >= 1 && D <= 3, "ntt schedule: degree must be 1, 2 or 3");
+  WHIR_REQUIRE(logC >= 0 && logC <= logN && logC <= 2 * MAX_PASS_LOG, "ntt schedule: bad coset size");
+  WHIR_REQUIRE(M > 0, "ntt schedule: no messages");
+  WHIR_REQUIRE(v1 >= 0 && v1 < 4 && v2 >= 0 && v2 < 4, "ntt schedule: variant is a 2-bit mask");
+  const std::vector<int> tws = tile_candidates();
+  auto tile_ok = [&](int tw) { return std::find(tws.begin(), tws.end(), tw) != tws.end(); };
+  WHIR_REQUIRE(tile_ok(tw1) && tile_ok(tw2), "ntt schedule: tile size not available on this device");
+  if (logC <= MAX_PASS_LOG) {
+    WHIR_REQUIRE(logC1 == 0, "ntt schedule: single-pass shapes take log_c1 = 0");
+  } else {
+    WHIR_REQUIRE(logC1 >= logC - MAX_PASS_LOG && logC1 <= MAX_PASS_LOG, "ntt schedule: split out of range");
+  }
+  SchedEntry e;
+  e.s = NttSched{logC1, v1, v2, tw1, tw2};
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  g_sched[SchedKey{D, logN, logC, M}] = e;
+}
+
+void ntt_clear_schedules() {
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  g_sched.clear();
+}
+
+std::string ntt_schedules_json() {
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  std::string out = "[";
+  char buf[512];
+  bool first = true;
+  for (const auto& kv : g_sched) {
+    const SchedKey& k = kv.first;
+    const SchedEntry& e = kv.second;
+    snprintf(buf, sizeof buf,
+             "%s{\"degree\":%d,\"log_n\":%d,\"log_c\":%d,\"messages\":%llu,\"log_c1\":%d,"
+             "\"pass1\":{\"pad\":%d,\"radix16\":%d,\"tile_words\":%d},"
+             "\"pass2\":{\"pad\":%d,\"radix16\":%d,\"tile_words\":%d},"
+             "\"tuned_ms\":%.4f,\"default_ms\":%.4f,\"candidates\":%d}",
+             first ? "" : ",", k.D, k.logN, k.logC, (unsigned long long)k.M, e.s.logC1, (e.s.v1 >> 1) & 1, e.s.v1 & 1,
+             e.s.tw1, (e.s.v2 >> 1) & 1, e.s.v2 & 1, e.s.tw2, e.ms1 + e.ms2, e.ms_default, e.candidates);
+    out += buf;
+    first = false;
+  }
+  return out + "]";
 }
 
 void cols_to_rows(int D, const uint64_t* cw, uint64_t* out, size_t N, size_t M, hipStream_t st) {

@@ -212,6 +212,24 @@ class Executor:
         self._keep[jid.value] = (cfg, t, d_vector, kind)
         return jid.value
 
+    def submit_batch(self, cfg: Config, t: Transcript, d_vectors, points=None, evaluations=None) -> int:
+        """Several commitments of batch_size vectors proven together (as prove_batch);
+        evaluations: (num_points, num_vectors, degree)."""
+        D, nv = cfg.degree, cfg.num_variables
+        pts = np.ascontiguousarray(np.asarray(points if points is not None else [], dtype=np.uint64)).reshape(-1)
+        ev = np.ascontiguousarray(np.asarray(evaluations if evaluations is not None else [],
+                                             dtype=np.uint64)).reshape(-1)
+        npts = pts.size // (nv * D) if nv else 0
+        assert ev.size == npts * len(d_vectors) * D, "evaluations must be num_points x num_vectors x degree words"
+        pts = pts if pts.size else np.zeros(1, dtype=np.uint64)
+        ev = ev if ev.size else np.zeros(1, dtype=np.uint64)
+        ptrs = (vp * max(len(d_vectors), 1))(*[d.ptr for d in d_vectors])
+        jid = ctypes.c_uint64()
+        _call("whir_executor_submit_batch", self.h, cfg.h, t.h, ptrs, len(d_vectors), _p64(pts), npts, _p64(ev),
+              ctypes.byref(jid))
+        self._keep[jid.value] = (cfg, t, list(d_vectors), self.COMMIT_AND_PROVE)
+        return jid.value
+
     def wait(self, job_id: int):
         """Blocks until the job ends; returns ((submitted, started, finished) seconds,
         folding randomness (num_variables * degree words; empty for commit-only jobs)).
