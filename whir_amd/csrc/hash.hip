@@ -424,16 +424,17 @@ __global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3w(const uint64_t
   sel8(odd, leaf[2], leaf[3], send);
   dpp_xor1(send, x2);  // even: partner's leaf[2]; odd: partner's leaf[3]
   {
-    uint32_t own_a[8], own_b[8];
+    // Operands are selected per lane and every lane runs the same two compressions
+    // (a branch on `odd` would make the wave run both sides).
+    uint32_t own_a[8], own_b[8], lhs[8], rhs[8];
     sel8(odd, leaf[1], leaf[0], own_a);
     sel8(odd, leaf[3], leaf[2], own_b);
-    if (odd) {
-      b3_pair(x1, own_a, pa);
-      b3_pair(x2, own_b, pb);
-    } else {
-      b3_pair(own_a, x1, pa);
-      b3_pair(own_b, x2, pb);
-    }
+    sel8(odd, x1, own_a, lhs);
+    sel8(odd, own_a, x1, rhs);
+    b3_pair(lhs, rhs, pa);
+    sel8(odd, x2, own_b, lhs);
+    sel8(odd, own_b, x2, rhs);
+    b3_pair(lhs, rhs, pb);
   }
   const size_t i1 = l >> 1, ka = odd ? 1 : 0;
   store_digest(nodes + 32 * (N + R0 / 2 + 32 * ka + i1), pa);
@@ -444,8 +445,12 @@ __global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3w(const uint64_t
   uint32_t y[8], gp[8];
   sel8(hi, pa, pb, send);
   dpp_xor2(send, y);  // lanes with bit 1 clear receive the partner's pa, set: partner's pb
-  if (hi) b3_pair(y, pb, gp);
-  else b3_pair(pa, y, gp);
+  {
+    uint32_t lhs[8], rhs[8];
+    sel8(hi, y, pa, lhs);
+    sel8(hi, pb, y, rhs);
+    b3_pair(lhs, rhs, gp);
+  }
   store_digest(nodes + 32 * (N + N / 2 + R0 / 4 + 16 * (l & 3) + (l >> 2)), gp);
 }
 
