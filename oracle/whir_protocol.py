@@ -673,6 +673,18 @@ def mle_evaluate_base(D, vector, point_list):
     return O.multilinear_extend(D, lv, np.concatenate(point_list))
 
 
+@dataclass
+class Covector:
+    """Linear form given as an explicit covector (src/algebra/linear_form/covector.rs):
+    vector_size extension elements (D words each), evaluated as a dot product."""
+    vector: np.ndarray
+
+
+def covector_evaluate(D, form: Covector, vector):
+    """Evaluate::evaluate of a Covector on a base-field vector (covector.rs:40-45, mixed_dot)."""
+    return O.dot(D, np.asarray(form.vector, dtype=np.uint64).reshape(-1), O.lift(D, ext(1, vector)))
+
+
 def _row_dot(D, weights, row, Ds):
     lrow = O.lift(D, row) if Ds == 1 else row
     return O.dot(D, np.concatenate(weights), lrow)
@@ -693,8 +705,9 @@ def prove(t, cfg: WhirConfig, vector, witness: Witness, points: list, evaluation
 def prove_batch(t, cfg: WhirConfig, vectors: list, witnesses: list, points: list, evaluations: list,
                 check_invariants: bool = False):
     """whir::Config::prove, src/protocols/whir/prover.rs:50-316: vectors committed in
-    `witnesses` (batch_size each, in order), MultilinearExtension linear forms at
-    `points`; evaluations[p][j] = MLE of vector j at point p.  check_invariants
+    `witnesses` (batch_size each, in order); linear forms in `points`, each a
+    MultilinearExtension point (a list of num_variables elements) or a Covector;
+    evaluations[p][j] = form p applied to vector j.  check_invariants
     asserts dot(vector, covector) == the_sum where the reference debug-asserts it."""
     D, eng = cfg.D, cfg.hash_engine
     n = cfg.initial.vector_size
@@ -728,7 +741,11 @@ def prove_batch(t, cfg: WhirConfig, vectors: list, witnesses: list, points: list
     form_rlc, oods_rlc = crlc[:len(points)], crlc[len(points):]
     cov = np.zeros(n * D, dtype=np.uint64)
     for rc, pt in zip(form_rlc, points):
-        cov = O.eval_eq(D, cov, np.concatenate(pt), rc)
+        if isinstance(pt, Covector):  # Covector::accumulate, covector.rs:21-23
+            v = np.asarray(pt.vector, dtype=np.uint64).reshape(-1)
+            cov = O.add(D, cov, O.mul(D, v, np.tile(np.asarray(rc, dtype=np.uint64), n))).reshape(-1)
+        else:
+            cov = O.eval_eq(D, cov, np.concatenate(pt), rc)
     the_sum = np.zeros(D, dtype=np.uint64)
     for rc, row in zip(form_rlc, evaluations):
         the_sum = e_add(D, the_sum, e_mul(D, rc, _row_dot(D, vector_rlc, np.concatenate(row), D)))
@@ -866,7 +883,12 @@ def verify_batch(r, cfg: WhirConfig, num_commitments: int, evaluations: list, po
     # FinalClaim::verify with MLE linear forms
     rlc = np.zeros(D, dtype=np.uint64)
     for c, pt in zip(form_rlc, points):
-        rlc = e_add(D, rlc, e_mul(D, c, eq_mle(D, pt, evaluation_point)))
+        if isinstance(pt, Covector):  # Covector::mle_evaluate, covector.rs:17-19
+            val = O.multilinear_extend(D, np.asarray(pt.vector, dtype=np.uint64).reshape(-1),
+                                       np.concatenate(evaluation_point))
+        else:
+            val = eq_mle(D, pt, evaluation_point)
+        rlc = e_add(D, rlc, e_mul(D, c, val))
     vassert(np.array_equal(rlc, linear_form_rlc), "final claim")
     r.check_eof()
     return evaluation_point

@@ -232,3 +232,74 @@ def test_batch_tampered_evaluation_is_rejected():
     swapped = [[row[2], row[3], row[0], row[1]] for row in evals]
     with pytest.raises(WP.VerificationError):
         WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 2, swapped, points)
+
+
+# ---------------------------------------------------------------- Covector linear forms
+def covector_iota(n, D):
+    """The reference tests' covector: element i is the field element i
+    (whir/mod.rs:226-229, (0..1 << num_variables).map(EF::from))."""
+    v = np.zeros(n * D, dtype=np.uint64)
+    v[0::D] = np.arange(n, dtype=np.uint64)
+    return WP.Covector(v)
+
+
+def run_forms(nv, D, forms_spec, B=1, NW=1, seed=9):
+    """forms_spec: string of 'm' (MLE point) and 'c' (covector) in transcript order."""
+    p = WP.ProtocolParameters(security_level=32, pow_bits=0, initial_folding_factor=2, folding_factor=2,
+                              starting_log_inv_rate=1, batch_size=B)
+    cfg = WP.WhirConfig.new(1 << nv, p, D)
+    cfg.disable_pow()
+    pid, sid = WP.protocol_ids(f"forms {nv} {D} {forms_spec} {B} {NW}")
+    vecs = [rand_vec(seed + j, 1 << nv) for j in range(B * NW)]
+    rng = random.Random(seed)
+    forms = []
+    for k, f in enumerate(forms_spec):
+        if f == "m":
+            forms.append([np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)])
+        elif k == len(forms_spec) - 1:
+            forms.append(covector_iota(1 << nv, D))
+        else:
+            g = np.random.default_rng(seed + 100 + k)
+            forms.append(WP.Covector(g.integers(0, O.P, size=(1 << nv) * D, dtype=np.uint64)))
+    evals = [[WP.covector_evaluate(D, f, v) if isinstance(f, WP.Covector) else WP.mle_evaluate_base(D, v, f)
+              for v in vecs] for f in forms]
+    t = WP.Transcript(pid, sid)
+    ws = [WP.commit_batch(t, cfg, vecs[w * B:(w + 1) * B]) for w in range(NW)]
+    WP.prove_batch(t, cfg, vecs, ws, forms, evals, check_invariants=True)
+    WP.verify_batch(WP.TranscriptReader(pid, sid, bytes(t.narg), bytes(t.hints)), cfg, NW, evals, forms)
+    return cfg, pid, sid, bytes(t.narg), bytes(t.hints), vecs, forms, evals
+
+
+@pytest.mark.parametrize("spec", ["mmc", "c", "cm", "mcm", "cc"])
+@pytest.mark.parametrize("D", [2, 3])
+def test_covector_forms_roundtrip(spec, D):
+    run_forms(10, D, spec)
+
+
+def test_covector_forms_with_several_commitments():
+    run_forms(9, 2, "mc", B=2, NW=2)
+
+
+def test_covector_evaluate_is_the_mle_sum():
+    """Covector evaluation = sum_i c_i v_i; with c = eq(z, .) it is the MLE at z."""
+    nv, D = 8, 2
+    rng = random.Random(3)
+    v = rand_vec(4, 1 << nv)
+    z = [np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)]
+    eq = O.eval_eq(D, np.zeros((1 << nv) * D, dtype=np.uint64), np.concatenate(z), WP.e_one(D))
+    assert np.array_equal(WP.covector_evaluate(D, WP.Covector(eq), v), WP.mle_evaluate_base(D, v, z))
+
+
+def test_tampered_covector_claim_is_rejected():
+    cfg, pid, sid, narg, hints, vecs, forms, evals = run_forms(10, 2, "mmc")
+    bad = [list(r) for r in evals]
+    bad[2][0] = WP.e_add(2, bad[2][0], WP.e_one(2))
+    with pytest.raises(WP.VerificationError):
+        WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 1, bad, forms)
+    # a different covector with the claimed value unchanged is caught by the final claim
+    other = list(forms)
+    v = np.asarray(forms[2].vector, dtype=np.uint64).copy()
+    v[0] = (int(v[0]) + 1) % O.P
+    other[2] = WP.Covector(v)
+    with pytest.raises(WP.VerificationError):
+        WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 1, evals, other)

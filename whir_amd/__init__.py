@@ -32,6 +32,7 @@ P = 0xFFFFFFFF00000001
 U64_MAX = 2**64 - 1
 
 HASH_COPY, HASH_SHA2, HASH_KECCAK, HASH_SHA3, HASH_BLAKE3 = 0, 1, 2, 3, 4
+FORM_MLE, FORM_COVECTOR = 0, 1  # linear form kinds of whir_prove_forms_dev
 ENGINE_NAMES = {HASH_COPY: "copy", HASH_SHA2: "sha2", HASH_KECCAK: "keccak", HASH_SHA3: "sha3",
                 HASH_BLAKE3: "Blake3"}
 
@@ -115,6 +116,8 @@ SIGNATURES = {
     "whir_ntt_set_schedule": [ci, sz, sz, sz, ci, ci, ci, ci, ci],
     "whir_ntt_clear_schedules": [],
     "whir_ntt_schedules": [ctypes.c_char_p, sz, szp],
+    "whir_prove_forms_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(ci), sz, u64p,
+                             ctypes.POINTER(vp), u64p, u64p],
     "whir_commit_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)],
     "whir_prove_batch_dev": [vp, vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, u64p, sz, u64p, u64p],
     "whir_witness_free": [vp],

@@ -65,6 +65,9 @@ void lift_dev(int D, const uint64_t* src, size_t n, uint64_t* dst, hipStream_t s
 // mixed_scalar_mul_add (src/algebra/mod.rs:88-98): acc[i] += w * v[i], v in the base field.
 void mixed_scalar_mul_add_dev(int D, uint64_t* acc, const uint64_t* w, const uint64_t* v, size_t n, hipStream_t st);
 
+// scalar_mul_add (src/algebra/mod.rs:76-83): acc[i] += w * v[i], v in the extension.
+void scalar_mul_add_dev(int D, uint64_t* acc, const uint64_t* w, const uint64_t* v, size_t n, hipStream_t s);
+
 // dot (src/algebra/mod.rs:36) to dev_out[0..D).
 void dot_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* dev_out, hipStream_t st);
 

@@ -731,6 +731,23 @@ void mixed_scalar_mul_add_dev(int D, uint64_t* acc, const uint64_t* w, const uin
 }
 
 template <int D>
+__global__ void k_axpy_ext(uint64_t* acc, Ext<D> w, const uint64_t* v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    st<D>(acc, i, gl::eadd<D>(ld<D>(acc, i), gl::emul<D>(w, ld<D>(v, i))));
+}
+
+void scalar_mul_add_dev(int D, uint64_t* acc, const uint64_t* w, const uint64_t* v, size_t n, hipStream_t s) {
+  if (!n) return;
+  switch (D) {
+    case 1: k_axpy_ext<1><<<grid_for(n), RT, 0, s>>>(acc, host_ext<1>(w), v, n); break;
+    case 2: k_axpy_ext<2><<<grid_for(n), RT, 0, s>>>(acc, host_ext<2>(w), v, n); break;
+    case 3: k_axpy_ext<3><<<grid_for(n), RT, 0, s>>>(acc, host_ext<3>(w), v, n); break;
+    default: throw Error("scalar_mul_add: extension degree must be 1, 2 or 3");
+  }
+  WHIR_HIP(hipGetLastError());
+}
+
+template <int D>
 __global__ __launch_bounds__(RT) void k_dot(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* parts) {
   Ext<D> acc[1] = {gl::ext_zero<D>()};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -343,11 +343,14 @@ void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs,
 
 // whir::Config::prove, src/protocols/whir/prover.rs:50-316: vecs[0..V) are the
 // committed base-field vectors, witnesses ws[0..W) hold them batch_size at a time
-// in order (V = W * batch_size); evals[(k*V + j)*D] is vector j at point k.
+// in order (V = W * batch_size).  Linear form k is a MultilinearExtension (kinds
+// null or kinds[k] == WHIR_FORM_MLE; its point is the next one in `points`) or a
+// Covector (the next device vector in `covs`, vector_size elements of D words);
+// evals[(k*V + j)*D] is vector j under form k.
 template <int D>
 void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, size_t V,
-                const DevWitness* const* ws, size_t W, const uint64_t* points, size_t np, const uint64_t* evals,
-                uint64_t* eval_point_out, hipStream_t s) {
+                const DevWitness* const* ws, size_t W, const int* kinds, size_t np, const uint64_t* points,
+                const uint64_t* const* covs, const uint64_t* evals, uint64_t* eval_point_out, hipStream_t s) {
   const int eng = cfg.engine;
   const size_t B = cfg.initial.num_vectors;
   WHIR_REQUIRE(V > 0 && W > 0, "prove: no vectors");
@@ -412,11 +415,24 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
   std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
   const bool has_constraints = !crlc.empty();
   {
-    // cov = sum_k crlc_k eq(point_k, .) + sum_o crlc_(np+o) (1, x_o, x_o^2, ...): one pass.
+    // cov = sum_{MLE forms k} crlc_k eq(point_k, .) + sum_o crlc_(np+o) (1, x_o, x_o^2, ...)
+    // in one pass, then + crlc_k covector_k for the Covector forms (Covector::accumulate,
+    // linear_form/covector.rs:21-23); the order of the sums does not change the field values.
     std::vector<uint64_t> erlc, srlc;
-    for (size_t k = 0; k < np; k++) push<D>(erlc, crlc[k]);
+    std::vector<size_t> cov_forms;
+    size_t nmle = 0;
+    for (size_t k = 0; k < np; k++) {
+      if (kinds && kinds[k] == WHIR_FORM_COVECTOR) {
+        cov_forms.push_back(k);
+      } else {
+        push<D>(erlc, crlc[k]);
+        nmle++;
+      }
+    }
     for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
-    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), np, nv, srlc.data(), opts.data(), nood, s);
+    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), nmle, nv, srlc.data(), opts.data(), nood, s);
+    for (size_t q = 0; q < cov_forms.size(); q++)
+      scalar_mul_add_dev(D, cov.as<uint64_t>(), crlc[cov_forms[q]].c, covs[q], n, s);
   }
   E<D> the_sum = gl::ext_zero<D>();
   {
@@ -640,12 +656,23 @@ int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* ve
   return whir_commit_batch_dev(c, t, &vector, 1, out);
 }
 
-int whir_prove_batch_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,
-                         const whir_witness* const* witnesses, size_t nw, const uint64_t* points, size_t np,
-                         const uint64_t* evals, uint64_t* eval_point) {
+int whir_prove_forms_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,
+                         const whir_witness* const* witnesses, size_t nw, const int* kinds, size_t nforms,
+                         const uint64_t* points, const uint64_t* const* covectors, const uint64_t* evals,
+                         uint64_t* eval_point) {
   API_BEGIN
   WHIR_REQUIRE(c && t && vectors && witnesses, "prove: null argument");
-  WHIR_REQUIRE(np == 0 || (points && evals), "prove: null points or evaluations");
+  WHIR_REQUIRE(nforms == 0 || evals, "prove: null evaluations");
+  size_t nmle = 0, ncov = 0;
+  for (size_t k = 0; k < nforms; k++) {
+    const int kind = kinds ? kinds[k] : WHIR_FORM_MLE;
+    WHIR_REQUIRE(kind == WHIR_FORM_MLE || kind == WHIR_FORM_COVECTOR, "prove: unknown linear form kind");
+    if (kind == WHIR_FORM_MLE) nmle++;
+    else ncov++;
+  }
+  WHIR_REQUIRE(nmle == 0 || points, "prove: null points");
+  WHIR_REQUIRE(ncov == 0 || covectors, "prove: null covectors");
+  for (size_t q = 0; q < ncov; q++) WHIR_REQUIRE(covectors[q], "prove: null covector");
   std::vector<const DevWitness*> ws(nw);
   for (size_t i = 0; i < nw; i++) {
     WHIR_REQUIRE(witnesses[i], "prove: null witness");
@@ -654,11 +681,23 @@ int whir_prove_batch_dev(const whir_config* c, whir_transcript* t, const uint64_
   for (size_t j = 0; j < nvec; j++) WHIR_REQUIRE(vectors[j], "prove: null vector");
   hipStream_t s = thread_stream();
   switch (c->cfg.D) {
-    case 1: prove_impl<1>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
-    case 2: prove_impl<2>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
-    case 3: prove_impl<3>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, points, np, evals, eval_point, s); break;
+    case 1:
+      prove_impl<1>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, kinds, nforms, points, covectors, evals, eval_point, s);
+      break;
+    case 2:
+      prove_impl<2>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, kinds, nforms, points, covectors, evals, eval_point, s);
+      break;
+    case 3:
+      prove_impl<3>(c->cfg, *t->t, vectors, nvec, ws.data(), nw, kinds, nforms, points, covectors, evals, eval_point, s);
+      break;
   }
   API_END
+}
+
+int whir_prove_batch_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,
+                         const whir_witness* const* witnesses, size_t nw, const uint64_t* points, size_t np,
+                         const uint64_t* evals, uint64_t* eval_point) {
+  return whir_prove_forms_dev(c, t, vectors, nvec, witnesses, nw, nullptr, np, points, nullptr, evals, eval_point);
 }
 
 int whir_prove_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, const whir_witness* w,

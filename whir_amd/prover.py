@@ -164,6 +164,32 @@ def prove_batch(cfg: Config, t: Transcript, d_vectors, witnesses, points, evalua
     return out[: nv * D]
 
 
+def prove_forms(cfg: Config, t: Transcript, d_vectors, witnesses, forms, evaluations) -> np.ndarray:
+    """whir::Config::prove with linear forms in order: a DeviceBuffer is a Covector
+    (vector_size elements of degree words), anything else an MLE point
+    (num_variables * degree words).  evaluations: (num_forms, num_vectors, degree)."""
+    from . import FORM_COVECTOR, FORM_MLE
+    D, nv = cfg.degree, cfg.num_variables
+    kinds = [FORM_COVECTOR if isinstance(f, DeviceBuffer) else FORM_MLE for f in forms]
+    pts = [np.ascontiguousarray(np.asarray(f, dtype=np.uint64)).reshape(-1) for f in forms
+           if not isinstance(f, DeviceBuffer)]
+    assert all(p.size == nv * D for p in pts), "MLE points must be num_variables x degree words"
+    flat = np.concatenate(pts) if pts else np.zeros(1, dtype=np.uint64)
+    covs = [f for f in forms if isinstance(f, DeviceBuffer)]
+    ev = np.ascontiguousarray(np.asarray(evaluations, dtype=np.uint64)).reshape(-1)
+    assert ev.size == len(forms) * len(d_vectors) * D, "evaluations: num_forms x num_vectors values"
+    if ev.size == 0:
+        ev = np.zeros(1, dtype=np.uint64)
+    karr = (ci * max(len(kinds), 1))(*kinds)
+    vptrs = (vp * max(len(d_vectors), 1))(*[d.ptr for d in d_vectors])
+    wptrs = (vp * max(len(witnesses), 1))(*[w.h for w in witnesses])
+    cptrs = (vp * max(len(covs), 1))(*[c.ptr for c in covs])
+    out = np.zeros(max(nv * D, 1), dtype=np.uint64)
+    _call("whir_prove_forms_dev", cfg.h, t.h, vptrs, len(d_vectors), wptrs, len(witnesses), karr, len(forms),
+          _p64(flat), cptrs, _p64(ev), _p64(out))
+    return out[: nv * D]
+
+
 def mle_evaluate(degree: int, d_vector: DeviceBuffer, num_variables: int, point) -> np.ndarray:
     pt = np.ascontiguousarray(np.asarray(point, dtype=np.uint64)).reshape(-1)
     if pt.size == 0:
