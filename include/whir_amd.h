@@ -304,15 +304,16 @@ int whir_prove_batch_dev(const whir_config *cfg, whir_transcript *t, const uint6
                          const uint64_t *points, size_t num_points, const uint64_t *evaluations,
                          uint64_t *evaluation_point);
 /* Linear forms of whir::Config::prove (src/algebra/linear_form/): a
- * MultilinearExtension at a point (multilinear_extension.rs) or an explicit Covector
- * (covector.rs). */
+ * MultilinearExtension at a point (multilinear_extension.rs), an explicit Covector
+ * (covector.rs) or a UnivariateEvaluation at a point (univariate_evaluation.rs). */
 #define WHIR_FORM_MLE 0
 #define WHIR_FORM_COVECTOR 1
-/* whir_prove_batch_dev with linear forms of either kind, in the order the reference
- * receives them (kinds[k] for form k; kinds NULL = all MLE).  Each MLE form takes the
- * next point of `points` (num_variables elements); each Covector form the next device
- * vector of `covectors` (vector_size elements of degree words).  evaluations:
- * num_forms x num_vectors values. */
+#define WHIR_FORM_UNIVARIATE 2
+/* whir_prove_batch_dev with linear forms of any kind, in the order the reference
+ * receives them (kinds[k] for form k; kinds NULL = all MLE).  `points` holds, in form
+ * order, num_variables elements per MLE form and one element per univariate form;
+ * each Covector form takes the next device vector of `covectors` (vector_size
+ * elements of degree words).  evaluations: num_forms x num_vectors values. */
 int whir_prove_forms_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *const *vectors,
                          size_t num_vectors, const whir_witness *const *witnesses, size_t num_witnesses,
                          const int *kinds, size_t num_forms, const uint64_t *points,

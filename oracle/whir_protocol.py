@@ -680,6 +680,22 @@ class Covector:
     vector: np.ndarray
 
 
+@dataclass
+class Univariate:
+    """UnivariateEvaluation linear form (src/algebra/linear_form/univariate_evaluation.rs):
+    v -> sum_i v_i x^i at an extension point x (D words)."""
+    point: np.ndarray
+
+
+def form_evaluate(D, form, vector):
+    """Evaluate::evaluate of any of the three linear forms on a base-field vector."""
+    if isinstance(form, Covector):
+        return covector_evaluate(D, form, vector)
+    if isinstance(form, Univariate):
+        return O.univariate_evaluate(1, D, ext(1, vector), np.asarray(form.point, dtype=np.uint64))
+    return mle_evaluate_base(D, vector, form)
+
+
 def covector_evaluate(D, form: Covector, vector):
     """Evaluate::evaluate of a Covector on a base-field vector (covector.rs:40-45, mixed_dot)."""
     return O.dot(D, np.asarray(form.vector, dtype=np.uint64).reshape(-1), O.lift(D, ext(1, vector)))
@@ -744,6 +760,9 @@ def prove_batch(t, cfg: WhirConfig, vectors: list, witnesses: list, points: list
         if isinstance(pt, Covector):  # Covector::accumulate, covector.rs:21-23
             v = np.asarray(pt.vector, dtype=np.uint64).reshape(-1)
             cov = O.add(D, cov, O.mul(D, v, np.tile(np.asarray(rc, dtype=np.uint64), n))).reshape(-1)
+        elif isinstance(pt, Univariate):  # UnivariateEvaluation::accumulate (geometric_accumulate)
+            cov = O.geometric_accumulate(D, cov, np.asarray(rc, dtype=np.uint64),
+                                         np.asarray(pt.point, dtype=np.uint64))
         else:
             cov = O.eval_eq(D, cov, np.concatenate(pt), rc)
     the_sum = np.zeros(D, dtype=np.uint64)
@@ -886,6 +905,8 @@ def verify_batch(r, cfg: WhirConfig, num_commitments: int, evaluations: list, po
         if isinstance(pt, Covector):  # Covector::mle_evaluate, covector.rs:17-19
             val = O.multilinear_extend(D, np.asarray(pt.vector, dtype=np.uint64).reshape(-1),
                                        np.concatenate(evaluation_point))
+        elif isinstance(pt, Univariate):  # univariate_evaluation.rs:46-56
+            val = univariate_mle(D, np.asarray(pt.point, dtype=np.uint64), evaluation_point)
         else:
             val = eq_mle(D, pt, evaluation_point)
         rlc = e_add(D, rlc, e_mul(D, c, val))

@@ -1,5 +1,5 @@
-"""Covector linear forms (src/algebra/linear_form/covector.rs) next to
-MultilinearExtension points, in the reference tests' shapes (whir/mod.rs:215-231:
+"""Covector and UnivariateEvaluation linear forms (src/algebra/linear_form/) next
+to MultilinearExtension points, in the reference tests' shapes (whir/mod.rs:215-231:
 MLE points then the covector (0, 1, 2, ...)): GPU proofs equal the oracle's byte for
 byte and its verifier accepts them."""
 import random
@@ -22,6 +22,8 @@ def make(nv, D, spec, B, NW, seed):
     for k, f in enumerate(spec):
         if f == "m":
             forms.append([np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)])
+        elif f == "u":
+            forms.append(WP.Univariate(np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64)))
         elif k == len(spec) - 1:
             v = np.zeros((1 << nv) * D, dtype=np.uint64)
             v[0::D] = np.arange(1 << nv, dtype=np.uint64)
@@ -29,13 +31,12 @@ def make(nv, D, spec, B, NW, seed):
         else:
             forms.append(WP.Covector(np.random.default_rng(seed + 50 + k).integers(0, O.P, size=(1 << nv) * D,
                                                                                      dtype=np.uint64)))
-    evals = [[WP.covector_evaluate(D, f, v) if isinstance(f, WP.Covector) else WP.mle_evaluate_base(D, v, f)
-              for v in vecs] for f in forms]
+    evals = [[WP.form_evaluate(D, f, v) for v in vecs] for f in forms]
     return vecs, forms, evals
 
 
 CASES = [(12, 2, "mmc", 1, 1), (10, 3, "c", 1, 1), (11, 2, "cm", 1, 1), (10, 2, "mcm", 2, 2),
-         (14, 2, "mc", 1, 2), (9, 1, "cc", 3, 1)]
+         (14, 2, "mc", 1, 2), (9, 1, "cc", 3, 1), (10, 2, "umc", 1, 1), (11, 3, "u", 2, 1), (12, 2, "mucu", 1, 2)]
 
 
 @pytest.mark.parametrize("nv,D,spec,B,NW", CASES)
@@ -53,7 +54,7 @@ def test_gpu_covector_forms_equal_oracle(nv, D, spec, B, NW):
     cfg = GP.Config(1 << nv, GP.ProtocolParameters(degree=D, **params)).disable_pow()
     dvs = [W.DeviceBuffer.from_array(v) for v in vecs]
     gforms = [W.DeviceBuffer.from_array(np.asarray(f.vector, dtype=np.uint64)) if isinstance(f, WP.Covector)
-              else np.concatenate(f) for f in forms]
+              else GP.Univariate(f.point) if isinstance(f, WP.Univariate) else np.concatenate(f) for f in forms]
     gt = GP.Transcript(pid, sid)
     gws = [GP.commit_batch(cfg, gt, dvs[w * B:(w + 1) * B]) for w in range(NW)]
     gev = np.array([[np.asarray(e, dtype=np.uint64) for e in row] for row in evals], dtype=np.uint64)

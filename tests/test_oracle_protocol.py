@@ -256,13 +256,14 @@ def run_forms(nv, D, forms_spec, B=1, NW=1, seed=9):
     for k, f in enumerate(forms_spec):
         if f == "m":
             forms.append([np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)])
+        elif f == "u":
+            forms.append(WP.Univariate(np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64)))
         elif k == len(forms_spec) - 1:
             forms.append(covector_iota(1 << nv, D))
         else:
             g = np.random.default_rng(seed + 100 + k)
             forms.append(WP.Covector(g.integers(0, O.P, size=(1 << nv) * D, dtype=np.uint64)))
-    evals = [[WP.covector_evaluate(D, f, v) if isinstance(f, WP.Covector) else WP.mle_evaluate_base(D, v, f)
-              for v in vecs] for f in forms]
+    evals = [[WP.form_evaluate(D, f, v) for v in vecs] for f in forms]
     t = WP.Transcript(pid, sid)
     ws = [WP.commit_batch(t, cfg, vecs[w * B:(w + 1) * B]) for w in range(NW)]
     WP.prove_batch(t, cfg, vecs, ws, forms, evals, check_invariants=True)
@@ -270,7 +271,7 @@ def run_forms(nv, D, forms_spec, B=1, NW=1, seed=9):
     return cfg, pid, sid, bytes(t.narg), bytes(t.hints), vecs, forms, evals
 
 
-@pytest.mark.parametrize("spec", ["mmc", "c", "cm", "mcm", "cc"])
+@pytest.mark.parametrize("spec", ["mmc", "c", "cm", "mcm", "cc", "u", "mu", "umc", "cuu"])
 @pytest.mark.parametrize("D", [2, 3])
 def test_covector_forms_roundtrip(spec, D):
     run_forms(10, D, spec)
@@ -303,3 +304,17 @@ def test_tampered_covector_claim_is_rejected():
     other[2] = WP.Covector(v)
     with pytest.raises(WP.VerificationError):
         WP.verify_batch(WP.TranscriptReader(pid, sid, narg, hints), cfg, 1, evals, other)
+
+
+def test_univariate_form_mle_is_the_mle_of_its_powers():
+    """UnivariateEvaluation::mle_evaluate (univariate_evaluation.rs:46-56) equals the
+    multilinear extension of (1, x, x^2, ...) at the point, variable 0 the MSB."""
+    nv, D = 7, 2
+    rng = random.Random(12)
+    x = np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64)
+    r = [np.array([rng.randrange(O.P) for _ in range(D)], dtype=np.uint64) for _ in range(nv)]
+    powers = np.concatenate(WP.geometric_sequence(D, x, 1 << nv))
+    assert np.array_equal(WP.univariate_mle(D, x, r), O.multilinear_extend(D, powers, np.concatenate(r)))
+    # and its evaluation on a vector is Horner's rule
+    v = rand_vec(13, 1 << nv)
+    assert np.array_equal(WP.form_evaluate(D, WP.Univariate(x), v), O.dot(D, powers, O.lift(D, v)))

@@ -32,7 +32,7 @@ P = 0xFFFFFFFF00000001
 U64_MAX = 2**64 - 1
 
 HASH_COPY, HASH_SHA2, HASH_KECCAK, HASH_SHA3, HASH_BLAKE3 = 0, 1, 2, 3, 4
-FORM_MLE, FORM_COVECTOR = 0, 1  # linear form kinds of whir_prove_forms_dev
+FORM_MLE, FORM_COVECTOR, FORM_UNIVARIATE = 0, 1, 2  # linear form kinds of whir_prove_forms_dev
 ENGINE_NAMES = {HASH_COPY: "copy", HASH_SHA2: "sha2", HASH_KECCAK: "keccak", HASH_SHA3: "sha3",
                 HASH_BLAKE3: "Blake3"}
 

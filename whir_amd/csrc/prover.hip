@@ -344,9 +344,10 @@ void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs,
 // whir::Config::prove, src/protocols/whir/prover.rs:50-316: vecs[0..V) are the
 // committed base-field vectors, witnesses ws[0..W) hold them batch_size at a time
 // in order (V = W * batch_size).  Linear form k is a MultilinearExtension (kinds
-// null or kinds[k] == WHIR_FORM_MLE; its point is the next one in `points`) or a
-// Covector (the next device vector in `covs`, vector_size elements of D words);
-// evals[(k*V + j)*D] is vector j under form k.
+// null or kinds[k] == WHIR_FORM_MLE; it takes the next num_variables elements of
+// `points`), a UnivariateEvaluation (WHIR_FORM_UNIVARIATE; the next element of
+// `points`) or a Covector (the next device vector in `covs`, vector_size elements
+// of D words); evals[(k*V + j)*D] is vector j under form k.
 template <int D>
 void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, size_t V,
                 const DevWitness* const* ws, size_t W, const int* kinds, size_t np, const uint64_t* points,
@@ -418,19 +419,31 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     // cov = sum_{MLE forms k} crlc_k eq(point_k, .) + sum_o crlc_(np+o) (1, x_o, x_o^2, ...)
     // in one pass, then + crlc_k covector_k for the Covector forms (Covector::accumulate,
     // linear_form/covector.rs:21-23); the order of the sums does not change the field values.
-    std::vector<uint64_t> erlc, srlc;
+    // UnivariateEvaluation forms join the OOD terms (UnivariateEvaluation::accumulate,
+    // linear_form/univariate_evaluation.rs:60-65, is the same geometric sum).
+    std::vector<uint64_t> erlc, srlc, mle_pts, geo_pts;
     std::vector<size_t> cov_forms;
     size_t nmle = 0;
+    const uint64_t* pp = points;
     for (size_t k = 0; k < np; k++) {
-      if (kinds && kinds[k] == WHIR_FORM_COVECTOR) {
+      const int kind = kinds ? kinds[k] : WHIR_FORM_MLE;
+      if (kind == WHIR_FORM_COVECTOR) {
         cov_forms.push_back(k);
+      } else if (kind == WHIR_FORM_UNIVARIATE) {
+        push<D>(srlc, crlc[k]);
+        geo_pts.insert(geo_pts.end(), pp, pp + D);
+        pp += D;
       } else {
         push<D>(erlc, crlc[k]);
+        mle_pts.insert(mle_pts.end(), pp, pp + nv * D);
+        pp += nv * D;
         nmle++;
       }
     }
     for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
-    covector_dev(D, cov.as<uint64_t>(), n, false, points, erlc.data(), nmle, nv, srlc.data(), opts.data(), nood, s);
+    geo_pts.insert(geo_pts.end(), opts.begin(), opts.end());
+    covector_dev(D, cov.as<uint64_t>(), n, false, mle_pts.data(), erlc.data(), nmle, nv, srlc.data(),
+                 geo_pts.data(), srlc.size() / D, s);
     for (size_t q = 0; q < cov_forms.size(); q++)
       scalar_mul_add_dev(D, cov.as<uint64_t>(), crlc[cov_forms[q]].c, covs[q], n, s);
   }
@@ -666,9 +679,10 @@ int whir_prove_forms_dev(const whir_config* c, whir_transcript* t, const uint64_
   size_t nmle = 0, ncov = 0;
   for (size_t k = 0; k < nforms; k++) {
     const int kind = kinds ? kinds[k] : WHIR_FORM_MLE;
-    WHIR_REQUIRE(kind == WHIR_FORM_MLE || kind == WHIR_FORM_COVECTOR, "prove: unknown linear form kind");
-    if (kind == WHIR_FORM_MLE) nmle++;
-    else ncov++;
+    WHIR_REQUIRE(kind == WHIR_FORM_MLE || kind == WHIR_FORM_COVECTOR || kind == WHIR_FORM_UNIVARIATE,
+                 "prove: unknown linear form kind");
+    if (kind == WHIR_FORM_COVECTOR) ncov++;
+    else nmle++;  // takes words from `points`
   }
   WHIR_REQUIRE(nmle == 0 || points, "prove: null points");
   WHIR_REQUIRE(ncov == 0 || covectors, "prove: null covectors");

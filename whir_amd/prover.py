@@ -164,16 +164,31 @@ def prove_batch(cfg: Config, t: Transcript, d_vectors, witnesses, points, evalua
     return out[: nv * D]
 
 
+class Univariate:
+    """UnivariateEvaluation linear form at an extension point x (degree words)."""
+
+    def __init__(self, x):
+        self.x = np.ascontiguousarray(np.asarray(x, dtype=np.uint64)).reshape(-1)
+
+
 def prove_forms(cfg: Config, t: Transcript, d_vectors, witnesses, forms, evaluations) -> np.ndarray:
     """whir::Config::prove with linear forms in order: a DeviceBuffer is a Covector
-    (vector_size elements of degree words), anything else an MLE point
-    (num_variables * degree words).  evaluations: (num_forms, num_vectors, degree)."""
-    from . import FORM_COVECTOR, FORM_MLE
+    (vector_size elements of degree words), a Univariate a UnivariateEvaluation,
+    anything else an MLE point (num_variables * degree words).  evaluations:
+    (num_forms, num_vectors, degree)."""
+    from . import FORM_COVECTOR, FORM_MLE, FORM_UNIVARIATE
     D, nv = cfg.degree, cfg.num_variables
-    kinds = [FORM_COVECTOR if isinstance(f, DeviceBuffer) else FORM_MLE for f in forms]
-    pts = [np.ascontiguousarray(np.asarray(f, dtype=np.uint64)).reshape(-1) for f in forms
-           if not isinstance(f, DeviceBuffer)]
-    assert all(p.size == nv * D for p in pts), "MLE points must be num_variables x degree words"
+    kinds = [FORM_COVECTOR if isinstance(f, DeviceBuffer) else FORM_UNIVARIATE if isinstance(f, Univariate)
+             else FORM_MLE for f in forms]
+    pts = []
+    for f in forms:
+        if isinstance(f, Univariate):
+            assert f.x.size == D, "univariate points are degree words"
+            pts.append(f.x)
+        elif not isinstance(f, DeviceBuffer):
+            p = np.ascontiguousarray(np.asarray(f, dtype=np.uint64)).reshape(-1)
+            assert p.size == nv * D, "MLE points must be num_variables x degree words"
+            pts.append(p)
     flat = np.concatenate(pts) if pts else np.zeros(1, dtype=np.uint64)
     covs = [f for f in forms if isinstance(f, DeviceBuffer)]
     ev = np.ascontiguousarray(np.asarray(evaluations, dtype=np.uint64)).reshape(-1)
