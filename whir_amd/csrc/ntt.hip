@@ -303,6 +303,7 @@ __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS
   const int stride = 1 << (logSub - LOGR);                 // S'/R
   const int groups = cols << (logS - LOGR);                // cols * S / R
   const int tw_shift = logS - logSub;                      // w_S' = w_S^(S/S')
+#pragma unroll 2
   for (int q = threadIdx.x; q < groups; q += blockDim.x) {
     const int base = q & (stride - 1);
     const int blk = q >> (logSub - LOGR);                  // col * (S/S') + block within column
