@@ -15,6 +15,8 @@
 //                  result is the globally minimal nonce.
 #include "common.h"
 #include "hash.h"
+
+#include <type_traits>
 #include "hash_dev.h"
 
 #include <algorithm>
@@ -38,29 +40,49 @@ __device__ __forceinline__ void store_digest(uint8_t* out, const uint32_t h[8]) 
   o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
-__global__ void k_hash_many(int engine, size_t size, const uint8_t* __restrict__ in, size_t count,
-                            uint8_t* __restrict__ out) {
+// Calls f(std::integral_constant<int, E>{}) for the engine E given at run time, so
+// each kernel is instantiated per engine (see hash_msg_t).
+template <class F>
+static void engine_switch(int engine, F&& f) {
+  switch (engine) {
+    case ENG_COPY: f(std::integral_constant<int, ENG_COPY>{}); break;
+    case ENG_SHA2: f(std::integral_constant<int, ENG_SHA2>{}); break;
+    case ENG_KECCAK: f(std::integral_constant<int, ENG_KECCAK>{}); break;
+    case ENG_SHA3: f(std::integral_constant<int, ENG_SHA3>{}); break;
+    case ENG_BLAKE3: f(std::integral_constant<int, ENG_BLAKE3>{}); break;
+    default: throw Error("unknown hash engine");
+  }
+}
+
+template <int ENG>
+__global__ void k_hash_many(size_t size, const uint8_t* __restrict__ in, size_t count, uint8_t* __restrict__ out) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= count) return;
   hd::MemReader rd{in + i * size, size};
   uint32_t h[8];
-  hd::hash_msg(engine, rd, size, h);
+  hd::hash_msg_t<ENG>(rd, size, h);
   store_digest(out + 32 * i, h);
 }
 
 void hash_many_dev(int engine, size_t size, const uint8_t* in, size_t count, uint8_t* out, hipStream_t st) {
   WHIR_REQUIRE(engine_supports_size(engine, size), "hash_many: engine does not support this message size");
   if (!count) return;
-  k_hash_many<<<blocks_for(count, 128), 128, 0, st>>>(engine, size, in, count, out);
+  engine_switch(engine, [&](auto E) {
+    k_hash_many<decltype(E)::value><<<blocks_for(count, 128), 128, 0, st>>>(size, in, count, out);
+  });
   WHIR_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ leaves
 static constexpr int LEAF_THREADS = 128;
+static size_t LEAF_TILE_MAX = [] {
+  const char* e = getenv("WHIR_LEAF_TILE_MAX");
+  return (size_t)(e ? atol(e) : 20480);
+}();
 
-template <int D>
-__global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(int engine, const uint64_t* __restrict__ cw, size_t N,
-                                                              size_t M, uint8_t* __restrict__ leaves, int RB) {
+template <int D, int ENG>
+__global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(const uint64_t* __restrict__ cw, size_t N, size_t M,
+                                                              uint8_t* __restrict__ leaves, int RB) {
   extern __shared__ uint64_t srow[];
   const size_t W = M * D;          // words per row
   const size_t stride = W + 1;     // padded LDS row stride (odd: spreads banks)
@@ -76,14 +98,14 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(int engine, const 
   for (size_t r = threadIdx.x; r < nrows; r += blockDim.x) {
     hd::WordReader rd{srow + r * stride, 8 * W};
     uint32_t h[8];
-    hd::hash_msg(engine, rd, 8 * W, h);
+    hd::hash_msg_t<ENG>(rd, 8 * W, h);
     store_digest(leaves + 32 * (r0 + r), h);
   }
 }
 
 // Rows wider than the LDS tile: one lane per row reading straight from HBM.
-template <int D>
-__global__ void k_leaves_cols_wide(int engine, const uint64_t* __restrict__ cw, size_t N, size_t M,
+template <int D, int ENG>
+__global__ void k_leaves_cols_wide(const uint64_t* __restrict__ cw, size_t N, size_t M,
                                    uint8_t* __restrict__ leaves) {
   struct ColReader {
     const uint64_t* cw;
@@ -99,7 +121,7 @@ __global__ void k_leaves_cols_wide(int engine, const uint64_t* __restrict__ cw, 
   if (r >= N) return;
   ColReader rd{cw, N, r, 8 * M * D};
   uint32_t h[8];
-  hd::hash_msg(engine, rd, 8 * M * D, h);
+  hd::hash_msg_t<ENG>(rd, 8 * M * D, h);
   store_digest(leaves + 32 * r, h);
 }
 
@@ -107,14 +129,20 @@ template <int D>
 static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st) {
   size_t W = M * D;
   int RB = (int)std::min<size_t>(LEAF_THREADS, 8192 / (W + 1));
+  // The LDS-staged kernel coalesces the row reads, but a tile above ~20 KiB per two
+  // waves caps residency; wider rows read their columns straight from HBM instead.
+  if ((size_t)RB * (W + 1) * 8 > LEAF_TILE_MAX) RB = 0;
   ProfScope prof(kernel_name(RB >= 16 ? "k_leaves_cols" : "k_leaves_cols_wide", D), st,
                  8.0 * (double)W * N + 32.0 * N);
-  if (RB >= 16) {
-    size_t smem = (size_t)RB * (W + 1) * 8;
-    k_leaves_cols<D><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(engine, cw, N, M, leaves, RB);
-  } else {
-    k_leaves_cols_wide<D><<<blocks_for(N, 128), 128, 0, st>>>(engine, cw, N, M, leaves);
-  }
+  engine_switch(engine, [&](auto E) {
+    constexpr int ENG = decltype(E)::value;
+    if (RB >= 16) {
+      size_t smem = (size_t)RB * (W + 1) * 8;
+      k_leaves_cols<D, ENG><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(cw, N, M, leaves, RB);
+    } else {
+      k_leaves_cols_wide<D, ENG><<<blocks_for(N, 128), 128, 0, st>>>(cw, N, M, leaves);
+    }
+  });
   WHIR_HIP(hipGetLastError());
 }
 
@@ -184,21 +212,23 @@ __device__ __forceinline__ void b3_pair(const uint32_t l[8], const uint32_t r[8]
   hd::b3_compress(out, m, 64, 0, hd::B3_CHUNK_START | hd::B3_CHUNK_END | hd::B3_ROOT);
 }
 
-__device__ __forceinline__ void hash_pair(int engine, const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
-  if (engine == ENG_BLAKE3) {
+template <int ENG>
+__device__ __forceinline__ void hash_pair(const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
+  if constexpr (ENG == ENG_BLAKE3) {
     b3_pair(l, r, out);
   } else {
     PairReader rd{l, r};
-    hd::hash_msg(engine, rd, 64, out);
+    hd::hash_msg_t<ENG>(rd, 64, out);
   }
 }
 
 // Reduce G levels of 2^G consecutive nodes held in h (node index `first` within a
 // level of width `width`), storing every node made at its place (the level of width
 // x starts at node 2W - 2x); the subtree root ends in h[0].
-template <int G>
-__device__ __forceinline__ void reduce_levels(int engine, uint32_t (&h)[8][8], uint8_t* nodes, size_t W,
-                                              size_t width, size_t first) {
+template <int G, int ENG, int NH>
+__device__ __forceinline__ void reduce_levels(uint32_t (&h)[NH][8], uint8_t* nodes, size_t W, size_t width,
+                                              size_t first) {
+  static_assert((1 << G) <= NH, "register group too small");
 #pragma unroll
   for (int lv = 0; lv < G; lv++) {
     width >>= 1;
@@ -207,7 +237,7 @@ __device__ __forceinline__ void reduce_levels(int engine, uint32_t (&h)[8][8], u
 #pragma unroll
     for (int k = 0; k < (1 << (G - 1 - lv)); k++) {
       uint32_t o[8];
-      hash_pair(engine, h[2 * k], h[2 * k + 1], o);
+      hash_pair<ENG>(h[2 * k], h[2 * k + 1], o);
       store_digest(nodes + 32 * (base + k), o);
 #pragma unroll
       for (int i = 0; i < 8; i++) h[k][i] = o[i];
@@ -221,8 +251,18 @@ __device__ __forceinline__ void reduce_levels(int engine, uint32_t (&h)[8][8], u
 // lane, every lane busy) to one in registers; after it, one level per phase (lane i
 // hashes the pair 2i, 2i + 1 from LDS), which keeps the chain of dependent hashes
 // short once few nodes are left.
-__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8_t* __restrict__ nodes, size_t W,
-                                                               size_t w, int L, int G1, uint32_t* root_out) {
+// Largest first-phase group (2^G nodes per lane).  Eight for every engine: for the
+// Keccak engines four-node groups (122 instead of 180 registers) measured 4% slower
+// on MI355X and pairs 40% slower, the longer chains of dependent hashes costing more
+// than the occupancy gained.
+template <int ENG>
+constexpr int merkle_gmax() { return 3; }
+static int merkle_gmax_rt(int) { return 3; }
+
+template <int ENG>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restrict__ nodes, size_t W, size_t w,
+                                                               int L, int G1, uint32_t* root_out) {
+  constexpr int NH = 1 << merkle_gmax<ENG>();
   __shared__ uint32_t buf[2][MERKLE_THREADS][8];
   size_t width = w;                                 // width of the current level
   size_t first = (size_t)blockIdx.x << L;           // block's first node within the current level
@@ -233,10 +273,10 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8
     const int G = from_global ? (left < G1 ? left : G1) : 1;
     const size_t lanes = cnt >> G;
     if (threadIdx.x < lanes) {
-      uint32_t h[8][8];
+      uint32_t h[NH][8];
       const size_t f = first + ((size_t)threadIdx.x << G);
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
+      for (int k = 0; k < NH; k++) {
         if (k < (1 << G)) {
           if (from_global) {
             const uint4* src = (const uint4*)(nodes + 32 * (2 * W - 2 * width + f + k));
@@ -249,9 +289,16 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(int engine, uint8
           }
         }
       }
-      if (G == 3) reduce_levels<3>(engine, h, nodes, W, width, f);
-      else if (G == 2) reduce_levels<2>(engine, h, nodes, W, width, f);
-      else reduce_levels<1>(engine, h, nodes, W, width, f);
+      if constexpr (NH >= 8) {
+        if (G == 3) reduce_levels<3, ENG, NH>(h, nodes, W, width, f);
+        else if (G == 2) reduce_levels<2, ENG, NH>(h, nodes, W, width, f);
+        else reduce_levels<1, ENG, NH>(h, nodes, W, width, f);
+      } else if constexpr (NH >= 4) {
+        if (G == 2) reduce_levels<2, ENG, NH>(h, nodes, W, width, f);
+        else reduce_levels<1, ENG, NH>(h, nodes, W, width, f);
+      } else {
+        reduce_levels<1, ENG, NH>(h, nodes, W, width, f);
+      }
 #pragma unroll
       for (int i = 0; i < 8; i++) buf[cur ^ 1][threadIdx.x][i] = h[0][i];
     }
@@ -277,16 +324,18 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
   while (w > 1) {
     // Wide levels: eight nodes per lane first (2^11 per block); otherwise pairs
     // (2^9 per block), which has the shortest chain of dependent hashes.
-    const int G1 = w >= ((size_t)MERKLE_THREADS << MERKLE_SPAN_LOG) ? 3 : 1;
-    const int maxL = G1 == 3 ? MERKLE_SPAN_LOG : MERKLE_SPAN_LOG - 2;
+    const int G1 = w >= ((size_t)MERKLE_THREADS << MERKLE_SPAN_LOG) ? merkle_gmax_rt(engine) : 1;
+    const int maxL = MERKLE_SPAN_LOG - 3 + G1;
     int L = 0;
     while (L < maxL && ((size_t)2 << L) <= w) L++;
     const size_t blocks = w >> L;
     const unsigned threads =
         (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
     ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)));
-    k_merkle_sub<<<(unsigned)blocks, threads, 0, st>>>(engine, nodes, W, w, L, G1,
-                                                       blocks == 1 ? (uint32_t*)root_out : nullptr);
+    engine_switch(engine, [&](auto E) {
+      k_merkle_sub<decltype(E)::value><<<(unsigned)blocks, threads, 0, st>>>(
+          nodes, W, w, L, G1, blocks == 1 ? (uint32_t*)root_out : nullptr);
+    });
     WHIR_HIP(hipGetLastError());
     w >>= L;
   }
@@ -493,10 +542,16 @@ void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, 
 static constexpr int POW_THREADS = 256;
 static constexpr int POW_PER_LANE = 16;
 static constexpr uint64_t POW_CHUNK = (uint64_t)POW_THREADS * POW_PER_LANE;
+static constexpr int POW_BLOCKS_PER_CU = 8;  // 16 and 32 measured no faster on MI355X
 
 // state[0] = minimal nonce found (init UINT64_MAX), state[1] = next chunk index.
-__global__ __launch_bounds__(POW_THREADS) void k_pow(int engine, uint4 ch_lo, uint4 ch_hi, uint64_t threshold,
-                                                     uint64_t first, uint64_t num_chunks, unsigned long long* state) {
+// A chunk is POW_THREADS * per_lane consecutive nonces (per_lane <= POW_PER_LANE, a
+// power of two, so chunks tile the POW_CHUNK-aligned window).
+template <int ENG>
+__global__ __launch_bounds__(POW_THREADS) void k_pow(uint4 ch_lo, uint4 ch_hi, uint64_t threshold,
+                                                     uint64_t first, uint64_t num_chunks, int per_lane,
+                                                     unsigned long long* state) {
+  const uint64_t chunk_len = (uint64_t)POW_THREADS * per_lane;
   __shared__ unsigned long long s_chunk;
   __shared__ int s_stop;
   const uint32_t chw[8] = {ch_lo.x, ch_lo.y, ch_lo.z, ch_lo.w, ch_hi.x, ch_hi.y, ch_hi.z, ch_hi.w};
@@ -505,7 +560,7 @@ __global__ __launch_bounds__(POW_THREADS) void k_pow(int engine, uint4 ch_lo, ui
       unsigned long long c = atomicAdd(&state[1], 1ull);
       unsigned long long best = __hip_atomic_load(&state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // Stop when the chunks run out or every remaining chunk lies above the current minimum.
-      s_stop = (c >= num_chunks) || (first + c * POW_CHUNK > best);
+      s_stop = (c >= num_chunks) || (first + c * chunk_len > best);
       s_chunk = c;
     }
     __syncthreads();
@@ -513,11 +568,11 @@ __global__ __launch_bounds__(POW_THREADS) void k_pow(int engine, uint4 ch_lo, ui
     const int stop = s_stop;
     __syncthreads();
     if (stop) break;
-    uint64_t start = first + chunk * POW_CHUNK;
-    for (int k = 0; k < POW_PER_LANE; k++) {
+    uint64_t start = first + chunk * chunk_len;
+    for (int k = 0; k < per_lane; k++) {
       uint64_t nonce = start + (uint64_t)k * POW_THREADS + threadIdx.x;
       uint32_t h[8];
-      if (engine == ENG_BLAKE3) {
+      if constexpr (ENG == ENG_BLAKE3) {
         uint32_t m[16];
 #pragma unroll
         for (int i = 0; i < 8; i++) m[i] = chw[i];
@@ -535,7 +590,7 @@ __global__ __launch_bounds__(POW_THREADS) void k_pow(int engine, uint4 ch_lo, ui
         words[4] = nonce;
         words[5] = words[6] = words[7] = 0;
         hd::WordReader rd{words, 64};
-        hd::hash_msg(engine, rd, 64, h);
+        hd::hash_msg_t<ENG>(rd, 64, h);
       }
       uint64_t v = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
       if (v <= threshold) atomicMin(&state[0], (unsigned long long)nonce);
@@ -565,11 +620,25 @@ bool pow_grind_dev(int engine, const uint8_t challenge[32], uint64_t threshold, 
   int dev = 0, cus = 0;
   WHIR_HIP(hipGetDevice(&dev));
   WHIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  uint64_t num_chunks = count / POW_CHUNK;
-  unsigned blocks = (unsigned)std::min<uint64_t>((uint64_t)cus * 8, num_chunks);
+  // Every block takes at least one chunk before the first minimum is known, so the
+  // nonces in flight are hashed whatever the difficulty.  Size them to the expected
+  // work (2^64 / (threshold + 1) hashes): a low-difficulty search (the query-phase
+  // PoW, 16-20 bits) runs one-nonce-per-lane chunks on fewer blocks instead of
+  // hashing millions of nonces it does not need.
+  const double expected = 18446744073709551616.0 / ((double)threshold + 1.0);
+  int per_lane = POW_PER_LANE;
+  uint64_t max_blocks = (uint64_t)cus * POW_BLOCKS_PER_CU;
+  while (per_lane > 1 && (double)max_blocks * POW_THREADS * per_lane > expected / 2) per_lane >>= 1;
+  while (max_blocks > (uint64_t)cus && (double)max_blocks * POW_THREADS * per_lane > expected / 2) max_blocks >>= 1;
+  const uint64_t chunk_len = (uint64_t)POW_THREADS * per_lane;
+  uint64_t num_chunks = count / chunk_len;
+  unsigned blocks = (unsigned)std::min<uint64_t>(max_blocks, num_chunks);
   ProfScope prof("pow", st, 0.0);
-  k_pow<<<blocks, POW_THREADS, 0, st>>>(engine, make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]),
-                                        threshold, first, num_chunks, state.as<unsigned long long>());
+  engine_switch(engine, [&](auto E) {
+    k_pow<decltype(E)::value><<<blocks, POW_THREADS, 0, st>>>(make_uint4(w[0], w[1], w[2], w[3]),
+                                                             make_uint4(w[4], w[5], w[6], w[7]), threshold, first,
+                                                             num_chunks, per_lane, state.as<unsigned long long>());
+  });
   WHIR_HIP(hipGetLastError());
   unsigned long long host = 0;
   download_sync(&host, state.ptr, 8, st);

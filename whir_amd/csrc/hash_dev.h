@@ -18,6 +18,27 @@ namespace hd {
 __device__ __forceinline__ uint32_t rotr32(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) { return (x << n) | (x >> ((64 - n) & 63)); }
 
+// 64-bit rotate left by a compile-time amount as two 32-bit funnel shifts
+// (v_alignbit_b32, full rate) instead of 64-bit shifts and ors.
+template <int N>
+__device__ __forceinline__ uint64_t rotl64c(uint64_t x) {
+  static_assert(N >= 0 && N < 64, "rotate amount");
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if constexpr (N >= 32) {
+    const uint32_t t = lo;
+    lo = hi;
+    hi = t;
+  }
+  constexpr int n = N & 31;
+  if constexpr (n == 0) {
+    return ((uint64_t)hi << 32) | lo;
+  } else {
+    const uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - n);  // (hi << n) | (lo >> (32 - n))
+    const uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - n);  // (lo << n) | (hi >> (32 - n))
+    return ((uint64_t)nhi << 32) | nlo;
+  }
+}
+
 // Reader over global memory (any alignment, any size).
 struct MemReader {
   const uint8_t* p;
@@ -187,34 +208,34 @@ __device__ __forceinline__ void keccak_f(uint64_t A[25]) {
     uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
     uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
     uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
-    uint64_t D0 = C4 ^ rotl64(C1, 1), D1 = C0 ^ rotl64(C2, 1), D2 = C1 ^ rotl64(C3, 1);
-    uint64_t D3 = C2 ^ rotl64(C4, 1), D4 = C3 ^ rotl64(C0, 1);
+    uint64_t D0 = C4 ^ rotl64c<1>(C1), D1 = C0 ^ rotl64c<1>(C2), D2 = C1 ^ rotl64c<1>(C3);
+    uint64_t D3 = C2 ^ rotl64c<1>(C4), D4 = C3 ^ rotl64c<1>(C0);
     // theta + rho + pi: B[y, 2x+3y] = rot(A[x, y] ^ D[x], r[x, y])
     uint64_t B0 = A[0] ^ D0;
-    uint64_t B10 = rotl64(A[1] ^ D1, 1);
-    uint64_t B20 = rotl64(A[2] ^ D2, 62);
-    uint64_t B5 = rotl64(A[3] ^ D3, 28);
-    uint64_t B15 = rotl64(A[4] ^ D4, 27);
-    uint64_t B16 = rotl64(A[5] ^ D0, 36);
-    uint64_t B1 = rotl64(A[6] ^ D1, 44);
-    uint64_t B11 = rotl64(A[7] ^ D2, 6);
-    uint64_t B21 = rotl64(A[8] ^ D3, 55);
-    uint64_t B6 = rotl64(A[9] ^ D4, 20);
-    uint64_t B7 = rotl64(A[10] ^ D0, 3);
-    uint64_t B17 = rotl64(A[11] ^ D1, 10);
-    uint64_t B2 = rotl64(A[12] ^ D2, 43);
-    uint64_t B12 = rotl64(A[13] ^ D3, 25);
-    uint64_t B22 = rotl64(A[14] ^ D4, 39);
-    uint64_t B23 = rotl64(A[15] ^ D0, 41);
-    uint64_t B8 = rotl64(A[16] ^ D1, 45);
-    uint64_t B18 = rotl64(A[17] ^ D2, 15);
-    uint64_t B3 = rotl64(A[18] ^ D3, 21);
-    uint64_t B13 = rotl64(A[19] ^ D4, 8);
-    uint64_t B14 = rotl64(A[20] ^ D0, 18);
-    uint64_t B24 = rotl64(A[21] ^ D1, 2);
-    uint64_t B9 = rotl64(A[22] ^ D2, 61);
-    uint64_t B19 = rotl64(A[23] ^ D3, 56);
-    uint64_t B4 = rotl64(A[24] ^ D4, 14);
+    uint64_t B10 = rotl64c<1>(A[1] ^ D1);
+    uint64_t B20 = rotl64c<62>(A[2] ^ D2);
+    uint64_t B5 = rotl64c<28>(A[3] ^ D3);
+    uint64_t B15 = rotl64c<27>(A[4] ^ D4);
+    uint64_t B16 = rotl64c<36>(A[5] ^ D0);
+    uint64_t B1 = rotl64c<44>(A[6] ^ D1);
+    uint64_t B11 = rotl64c<6>(A[7] ^ D2);
+    uint64_t B21 = rotl64c<55>(A[8] ^ D3);
+    uint64_t B6 = rotl64c<20>(A[9] ^ D4);
+    uint64_t B7 = rotl64c<3>(A[10] ^ D0);
+    uint64_t B17 = rotl64c<10>(A[11] ^ D1);
+    uint64_t B2 = rotl64c<43>(A[12] ^ D2);
+    uint64_t B12 = rotl64c<25>(A[13] ^ D3);
+    uint64_t B22 = rotl64c<39>(A[14] ^ D4);
+    uint64_t B23 = rotl64c<41>(A[15] ^ D0);
+    uint64_t B8 = rotl64c<45>(A[16] ^ D1);
+    uint64_t B18 = rotl64c<15>(A[17] ^ D2);
+    uint64_t B3 = rotl64c<21>(A[18] ^ D3);
+    uint64_t B13 = rotl64c<8>(A[19] ^ D4);
+    uint64_t B14 = rotl64c<18>(A[20] ^ D0);
+    uint64_t B24 = rotl64c<2>(A[21] ^ D1);
+    uint64_t B9 = rotl64c<61>(A[22] ^ D2);
+    uint64_t B19 = rotl64c<56>(A[23] ^ D3);
+    uint64_t B4 = rotl64c<14>(A[24] ^ D4);
     // chi
     A[0] = B0 ^ (~B1 & B2);   A[1] = B1 ^ (~B2 & B3);   A[2] = B2 ^ (~B3 & B4);
     A[3] = B3 ^ (~B4 & B0);   A[4] = B4 ^ (~B0 & B1);
@@ -247,8 +268,14 @@ __device__ void keccak256(const R& rd, size_t size, uint8_t pad, uint32_t out[8]
       A[i] ^= v;
     }
     if (b == nblocks - 1) {
-      size_t pp = size - bo;  // pad position within the block
-      A[pp >> 3] ^= (uint64_t)pad << (8 * (pp & 7));
+      // Padding: the domain byte at position pp of the block.  The word is chosen by
+      // comparing against every static index, so the state stays in registers (a
+      // runtime index into A would put it in scratch memory).
+      const size_t pp = size - bo;
+      const uint64_t padw = (uint64_t)pad << (8 * (pp & 7));
+#pragma unroll
+      for (int i = 0; i < 17; i++)
+        if ((size_t)i == (pp >> 3)) A[i] ^= padw;
       A[16] ^= 0x80ull << 56;
     }
     keccak_f(A);
@@ -331,7 +358,30 @@ __device__ void sha256(const R& rd, size_t size, uint32_t out[8]) {
 }
 
 // ------------------------------------------------------------------ dispatch
-// 32-byte digest as 8 little-endian words.
+// 32-byte digest as 8 little-endian words, engine fixed at compile time: a kernel
+// instantiated per engine holds one hash's code and is allocated its registers and
+// private memory only (the BLAKE3 multi-chunk stack, for one, is private memory).
+template <int ENG, class R>
+__device__ __forceinline__ void hash_msg_t(const R& rd, size_t size, uint32_t out[8]) {
+  if constexpr (ENG == 4) {
+    blake3(rd, size, out);
+  } else if constexpr (ENG == 3) {
+    keccak256(rd, size, 0x06, out);
+  } else if constexpr (ENG == 2) {
+    keccak256(rd, size, 0x01, out);
+  } else if constexpr (ENG == 1) {
+    sha256(rd, size, out);
+  } else {  // Copy: zero-padded input of at most 32 bytes (src/hash/copy_engine.rs:25-60)
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint32_t v = (4 * (size_t)i < size) ? rd.w32(4 * i) : 0u;
+      if (4 * (size_t)i < size && size < 4 * (size_t)i + 4) v &= (1u << (8 * (size & 3))) - 1;
+      out[i] = v;
+    }
+  }
+}
+
+// Same with the engine chosen at run time.
 template <class R>
 __device__ __forceinline__ void hash_msg(int engine, const R& rd, size_t size, uint32_t out[8]) {
   switch (engine) {

@@ -498,11 +498,11 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass1(RsArgs a, const uint6
   }
 }
 
-// Post factors held in registers per thread: 8 for Fp2 (tiles up to 2048
-// elements), where the extra registers cost no occupancy (measured 7% faster on
-// MI355X); none for Fp, where they would halve it, or Fp3, which has none spare.
+// Post factors held in registers per thread: 8 for Fp and Fp2 (tiles up to 2048
+// elements), measured 7-8% faster on MI355X than reloading them per message; none
+// for Fp3, which has no registers to spare.
 template <int D>
-constexpr int pass1_ept() { return D == 2 ? 8 : 0; }
+constexpr int pass1_ept() { return D <= 2 ? 8 : 0; }
 
 // --------------------------------------------------------------------------
 // Pass 2: rows.  Block = (b consecutive t2, coset c, message j).
@@ -619,6 +619,10 @@ static NttSched default_sched(int D, int logC) {
   NttSched s;
   s.logC1 = logC <= MAX_PASS_LOG ? 0 : (logC + 1) / 2;
   s.v1 = s.v2 = variant_bits(D);
+  // Fp pass 1 (post factors in registers) is faster padded with radix-8 steps than
+  // with radix 16 (1.13 vs 1.22 ms for both passes at 2^26 outputs on MI355X, as
+  // measured by the autotuner); pass 2 keeps radix 16.
+  if (D == 1 && !getenv("WHIR_NTT_VARIANT") && logC > MAX_PASS_LOG) s.v1 = 2;
   s.tw1 = s.tw2 = tile_words();
   return s;
 }
