@@ -201,54 +201,104 @@ __constant__ static const uint64_t KRC[24] = {
     0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800aull, 0x800000008000000aull,
     0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
 
-__device__ __forceinline__ void keccak_f(uint64_t A[25]) {
-  for (int r = 0; r < 24; r++) {
-    uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
-    uint64_t C1 = A[1] ^ A[6] ^ A[11] ^ A[16] ^ A[21];
-    uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
-    uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
-    uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
-    uint64_t D0 = C4 ^ rotl64c<1>(C1), D1 = C0 ^ rotl64c<1>(C2), D2 = C1 ^ rotl64c<1>(C3);
-    uint64_t D3 = C2 ^ rotl64c<1>(C4), D4 = C3 ^ rotl64c<1>(C0);
-    // theta + rho + pi: B[y, 2x+3y] = rot(A[x, y] ^ D[x], r[x, y])
-    uint64_t B0 = A[0] ^ D0;
-    uint64_t B10 = rotl64c<1>(A[1] ^ D1);
-    uint64_t B20 = rotl64c<62>(A[2] ^ D2);
-    uint64_t B5 = rotl64c<28>(A[3] ^ D3);
-    uint64_t B15 = rotl64c<27>(A[4] ^ D4);
-    uint64_t B16 = rotl64c<36>(A[5] ^ D0);
-    uint64_t B1 = rotl64c<44>(A[6] ^ D1);
-    uint64_t B11 = rotl64c<6>(A[7] ^ D2);
-    uint64_t B21 = rotl64c<55>(A[8] ^ D3);
-    uint64_t B6 = rotl64c<20>(A[9] ^ D4);
-    uint64_t B7 = rotl64c<3>(A[10] ^ D0);
-    uint64_t B17 = rotl64c<10>(A[11] ^ D1);
-    uint64_t B2 = rotl64c<43>(A[12] ^ D2);
-    uint64_t B12 = rotl64c<25>(A[13] ^ D3);
-    uint64_t B22 = rotl64c<39>(A[14] ^ D4);
-    uint64_t B23 = rotl64c<41>(A[15] ^ D0);
-    uint64_t B8 = rotl64c<45>(A[16] ^ D1);
-    uint64_t B18 = rotl64c<15>(A[17] ^ D2);
-    uint64_t B3 = rotl64c<21>(A[18] ^ D3);
-    uint64_t B13 = rotl64c<8>(A[19] ^ D4);
-    uint64_t B14 = rotl64c<18>(A[20] ^ D0);
-    uint64_t B24 = rotl64c<2>(A[21] ^ D1);
-    uint64_t B9 = rotl64c<61>(A[22] ^ D2);
-    uint64_t B19 = rotl64c<56>(A[23] ^ D3);
-    uint64_t B4 = rotl64c<14>(A[24] ^ D4);
-    // chi
-    A[0] = B0 ^ (~B1 & B2);   A[1] = B1 ^ (~B2 & B3);   A[2] = B2 ^ (~B3 & B4);
-    A[3] = B3 ^ (~B4 & B0);   A[4] = B4 ^ (~B0 & B1);
-    A[5] = B5 ^ (~B6 & B7);   A[6] = B6 ^ (~B7 & B8);   A[7] = B7 ^ (~B8 & B9);
-    A[8] = B8 ^ (~B9 & B5);   A[9] = B9 ^ (~B5 & B6);
-    A[10] = B10 ^ (~B11 & B12); A[11] = B11 ^ (~B12 & B13); A[12] = B12 ^ (~B13 & B14);
-    A[13] = B13 ^ (~B14 & B10); A[14] = B14 ^ (~B10 & B11);
-    A[15] = B15 ^ (~B16 & B17); A[16] = B16 ^ (~B17 & B18); A[17] = B17 ^ (~B18 & B19);
-    A[18] = B18 ^ (~B19 & B15); A[19] = B19 ^ (~B15 & B16);
-    A[20] = B20 ^ (~B21 & B22); A[21] = B21 ^ (~B22 & B23); A[22] = B22 ^ (~B23 & B24);
-    A[23] = B23 ^ (~B24 & B20); A[24] = B24 ^ (~B20 & B21);
-    A[0] ^= KRC[r];
+// Three-input XOR in one VALU instruction: v_bitop3_b32 with the parity truth table
+// 0x96 (symmetric, so the operand order does not matter); the compiler emits two
+// two-input XORs for a ^ b ^ c.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// 64-bit lane (lo, hi) rotated left by a compile-time amount: two funnel shifts.
+template <int N>
+__device__ __forceinline__ void rot2(uint32_t lo, uint32_t hi, uint32_t& olo, uint32_t& ohi) {
+  static_assert(N >= 0 && N < 64, "rotate amount");
+  if constexpr (N >= 32) {
+    const uint32_t t = lo;
+    lo = hi;
+    hi = t;
   }
+  constexpr int n = N & 31;
+  if constexpr (n == 0) {
+    olo = lo;
+    ohi = hi;
+  } else {
+    ohi = __builtin_amdgcn_alignbit(hi, lo, 32 - n);
+    olo = __builtin_amdgcn_alignbit(lo, hi, 32 - n);
+  }
+}
+
+// Keccak-f[1600] on the state as 32-bit halves (FIPS 202 round function).  Written
+// on 32-bit words so that chi, a ^ (~b & c), is one v_bitop3_b32 per half (on
+// 64-bit values the compiler splits it into a bit-field insert and an XOR) and the
+// column parities use three-input XORs.
+__device__ __forceinline__ void keccak_f(uint64_t A[25]) {
+  uint32_t L[25], H[25];
+#pragma unroll
+  for (int i = 0; i < 25; i++) {
+    L[i] = (uint32_t)A[i];
+    H[i] = (uint32_t)(A[i] >> 32);
+  }
+  for (int r = 0; r < 24; r++) {
+    uint32_t CL[5], CH[5], DL[5], DH[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+      CL[x] = xor3(xor3(L[x], L[x + 5], L[x + 10]), L[x + 15], L[x + 20]);
+      CH[x] = xor3(xor3(H[x], H[x + 5], H[x + 10]), H[x + 15], H[x + 20]);
+    }
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+      uint32_t rl, rh;
+      rot2<1>(CL[(x + 1) % 5], CH[(x + 1) % 5], rl, rh);
+      DL[x] = CL[(x + 4) % 5] ^ rl;
+      DH[x] = CH[(x + 4) % 5] ^ rh;
+    }
+    // theta + rho + pi: B[y, 2x+3y] = rot(A[x, y] ^ D[x], r[x, y])
+    uint32_t BL[25], BH[25];
+#define KECCAK_RP(src, dst, rot) rot2<rot>(L[src] ^ DL[(src) % 5], H[src] ^ DH[(src) % 5], BL[dst], BH[dst]);
+    KECCAK_RP(0, 0, 0)
+    KECCAK_RP(1, 10, 1)
+    KECCAK_RP(2, 20, 62)
+    KECCAK_RP(3, 5, 28)
+    KECCAK_RP(4, 15, 27)
+    KECCAK_RP(5, 16, 36)
+    KECCAK_RP(6, 1, 44)
+    KECCAK_RP(7, 11, 6)
+    KECCAK_RP(8, 21, 55)
+    KECCAK_RP(9, 6, 20)
+    KECCAK_RP(10, 7, 3)
+    KECCAK_RP(11, 17, 10)
+    KECCAK_RP(12, 2, 43)
+    KECCAK_RP(13, 12, 25)
+    KECCAK_RP(14, 22, 39)
+    KECCAK_RP(15, 23, 41)
+    KECCAK_RP(16, 8, 45)
+    KECCAK_RP(17, 18, 15)
+    KECCAK_RP(18, 3, 21)
+    KECCAK_RP(19, 13, 8)
+    KECCAK_RP(20, 14, 18)
+    KECCAK_RP(21, 24, 2)
+    KECCAK_RP(22, 9, 61)
+    KECCAK_RP(23, 19, 56)
+    KECCAK_RP(24, 4, 14)
+#undef KECCAK_RP
+    // chi
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
+#pragma unroll
+      for (int x = 0; x < 5; x++) {
+        L[y + x] = BL[y + x] ^ (~BL[y + (x + 1) % 5] & BL[y + (x + 2) % 5]);
+        H[y + x] = BH[y + x] ^ (~BH[y + (x + 1) % 5] & BH[y + (x + 2) % 5]);
+      }
+    }
+    // iota
+    const uint64_t rc = KRC[r];
+    L[0] ^= (uint32_t)rc;
+    H[0] ^= (uint32_t)(rc >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 25; i++) A[i] = ((uint64_t)H[i] << 32) | L[i];
 }
 
 // Keccak sponge with rate 136 and domain byte `pad` (0x06 SHA3-256, 0x01 Keccak-256).
