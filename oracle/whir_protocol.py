@@ -890,6 +890,9 @@ def verify_batch(r, cfg: WhirConfig, num_commitments: int, evaluations: list, po
     round_fr.append(ffr)
     evaluation_point = [x for rf in round_fr for x in rf]
     poly_eval = O.multilinear_extend(D, final_vector, np.concatenate(ffr)) if ffr else final_vector[:D]
+    # the_sum / poly_eval (verifier.rs:241-243): ark-ff panics on a zero divisor,
+    # which happens for the zero vector; reported here as a rejection.
+    vassert(np.asarray(poly_eval).any(), "final polynomial evaluates to zero (division by zero)")
     inv = _e_inv(D, poly_eval)
     linear_form_rlc = e_mul(D, the_sum, inv)
     nv_list = [cfg.initial.vector_size.bit_length() - 1] + [rc.irs.vector_size.bit_length() - 1 for rc in cfg.rounds]

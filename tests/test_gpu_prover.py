@@ -122,3 +122,27 @@ def test_gpu_prover_goldilocks3_sha3_with_pow_accepts(nv, npts):
     for k in range(npts):
         assert np.array_equal(gevals[k], WP.mle_evaluate_base(D, vec, points[k]))
     WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, gevals, points)
+
+
+@pytest.mark.parametrize("kind", ["zeros", "max", "iota", "ones"])
+@pytest.mark.parametrize("nv,D,k0,k,lir", [(12, 2, 4, 4, 1), (5, 3, 1, 1, 1), (4, 2, 4, 4, 2), (9, 1, 3, 2, 1)])
+def test_gpu_proof_special_vectors(kind, nv, D, k0, k, lir):
+    """Edge inputs: the all-zero vector (zero OOD values, zero sums), every
+    coefficient p - 1 (carries at every reduction), v[i] = i (the reference CLI's
+    input, src/bin/main.rs:150) and all ones (the reference tests' vector,
+    whir/mod.rs:209); tiny sizes down to one folding round and no WHIR rounds."""
+    n = 1 << nv
+    vec = {"zeros": np.zeros(n, np.uint64), "max": np.full(n, O.P - 1, np.uint64),
+           "iota": np.arange(n, dtype=np.uint64), "ones": np.ones(n, np.uint64)}[kind]
+    pid, sid, _, points, params = make_case(nv, D, W.HASH_BLAKE3, k0, k, lir, 2, ud=(D == 1))
+    cfg, evals, narg, hints, ep = oracle_proof(nv, D, pid, sid, vec, points, params, True)
+    gevals, gnarg, ghints, gep = gpu_proof(nv, D, pid, sid, vec, points, params, True)
+    assert (gnarg, ghints) == (narg, hints)
+    assert np.array_equal(gep, np.concatenate(ep))
+    if kind == "zeros":
+        # The verifier divides by the final polynomial's value (verifier.rs:241-243),
+        # zero here: the reference panics, the oracle rejects.
+        with pytest.raises(WP.VerificationError, match="division by zero"):
+            WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, evals, points)
+    else:
+        WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, evals, points)
