@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="arrival rate; 0 = 70%% of measured capacity")
+    ap.add_argument("--inputs", choices=("uniform", "cli"), default="uniform",
+                    help="uniform: seeded uniform coefficients and points; cli: the reference CLI's inputs "
+                         "(v[i] = i, points (x, ..., x) for x = 0, 1, ..., src/bin/main.rs:150,173-175)")
+    ap.add_argument("--linear-constraints", type=int, default=0,
+                    help="Covector constraints (0, 1, ..., n-1) before the points, as the reference CLI's "
+                         "--linear-constraints (src/bin/main.rs:163-170)")
     ap.add_argument("--ntt-autotune", type=int, default=1, choices=(0, 1),
                     help="tune each NTT shape's schedule during the warmup steps (1, default) or use the "
                          "built-in schedules (0)")
@@ -86,10 +92,35 @@ def main():
     cfg = GP.Config(1 << nv, params).disable_pow()
     desc = cfg.describe()
     rng = np.random.default_rng(20241202)
-    vec = uniform_field(rng, 1 << nv)
-    points = [uniform_field(rng, nv * D) for _ in range(2)]
+    n = 1 << nv
+    if args.inputs == "cli":
+        vec = np.arange(n, dtype=np.uint64)
+        points = []
+        for x in range(2):
+            pt = np.zeros(nv * D, dtype=np.uint64)
+            pt[0::D] = x
+            points.append(pt)
+    else:
+        vec = uniform_field(rng, n)
+        points = [uniform_field(rng, nv * D) for _ in range(2)]
     dvec = W.DeviceBuffer.from_array(vec)
-    evals = [GP.mle_evaluate(D, dvec, nv, pt) for pt in points]
+    # Covector constraints (0, 1, ..., n-1) embedded in the extension; their claimed
+    # values are dot products computed on the GPU.
+    cov_host = None
+    covs = []
+    cov_evals = []
+    if args.linear_constraints:
+        cov_host = np.zeros(n * D, dtype=np.uint64)
+        cov_host[0::D] = np.arange(n, dtype=np.uint64)
+        lifted = np.zeros(n * D, dtype=np.uint64)
+        lifted[0::D] = vec
+        ce = W.dot(D, cov_host, lifted)
+        del lifted
+        for _ in range(args.linear_constraints):
+            covs.append(W.DeviceBuffer.from_array(cov_host))
+            cov_evals.append(ce)
+    evals = cov_evals + [GP.mle_evaluate(D, dvec, nv, pt) for pt in points]
+    forms = covs + points
     flat_pts = np.concatenate(points)
     flat_ev = np.concatenate(evals)
     import hashlib
@@ -106,7 +137,10 @@ def main():
     def step():
         t = GP.Transcript(pid, sid)
         w = GP.commit(cfg, t, dvec)
-        GP.prove(cfg, t, dvec, w, flat_pts, flat_ev)
+        if covs:
+            GP.prove_forms(cfg, t, [dvec], [w], forms, np.array(evals, dtype=np.uint64).reshape(len(forms), 1, D))
+        else:
+            GP.prove(cfg, t, dvec, w, flat_pts, flat_ev)
         return t
 
     W.ntt_autotune(bool(args.ntt_autotune))
@@ -160,7 +194,8 @@ def main():
             starting_log_inv_rate=2, hash_engine=W.HASH_BLAKE3), D)
         ocfg.disable_pow()
         narg, hints = last.proof()
-        pts_list = [[p[i * D:(i + 1) * D] for i in range(nv)] for p in points]
+        pts_list = [WP.Covector(cov_host) for _ in covs] + [[p[i * D:(i + 1) * D] for i in range(nv)]
+                                                            for p in points]
         try:
             WP.verify(WP.TranscriptReader(pid, sid, narg, hints), ocfg, [e for e in evals], pts_list)
             verified = True
@@ -169,7 +204,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(nv, D, vec, points, evals, pid, sid, last.proof())
+        cpu = cpu_baseline(nv, D, vec, points, evals, pid, sid, last.proof(),
+                           [cov_host for _ in covs])
     pow_rates = None if args.no_pow else bench_pow(W, GP, nv)
     others = None if args.no_other_configs else bench_other_configs(W, GP)
     batched = None if args.no_batch else bench_batched(args.workers)
@@ -189,10 +225,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 (Goldilocks, exact)",
-        "data": "synthetic: 2^24 uniform Field64 coefficients, 2 uniform Field64_2 points (seed 20241202)",
+        "data": (f"synthetic: 2^{nv} uniform Field64 coefficients, 2 uniform Field64_2 points (seed 20241202)"
+                 if args.inputs == "uniform" else
+                 f"the reference CLI's inputs: v[i] = i (2^{nv} coefficients), points (0,..,0) and (1,..,1)")
+                + (f", {args.linear_constraints} covector constraint(s) (0, 1, ..., n-1)" if args.linear_constraints
+                   else ""),
         "config": {"workload": f"WHIR PCS Goldilocks2 d={nv} rate 1/4 fold 4 Blake3 100-bit list decoding, "
                                "2 evaluation points, PoW thresholds disabled", "num_variables": nv,
                    "log_inv_rate": 2, "folding_factor": 4, "hash": "Blake3", "evaluations": 2,
+                   "linear_constraints": args.linear_constraints, "inputs": args.inputs,
                    "parallelism": "single GPU"},
         "proofs_per_s": round(1e3 / ms, 3),
         "proof_bytes": len(last.proof()[0]) + len(last.proof()[1]),
@@ -459,7 +500,7 @@ def bench_serving(workers: int, seconds: float, qps: float):
             "identical_alone": same}
 
 
-def cpu_baseline(nv, D, vec, points, evals, pid, sid, proof):
+def cpu_baseline(nv, D, vec, points, evals, pid, sid, proof, covectors=()):
     """The oracle prover (oracle/whir_protocol.py over oracle/whir_oracle.c, its
     large loops split over all host threads OpenMP gives it) on the same instance
     as the timed GPU proof: same configuration, vector, points and transcript
@@ -471,7 +512,7 @@ def cpu_baseline(nv, D, vec, points, evals, pid, sid, proof):
                               unique_decoding=False, starting_log_inv_rate=2, hash_engine=O.ENGINE_BLAKE3)
     cfg = WP.WhirConfig.new(1 << nv, p, D)
     cfg.disable_pow()
-    pts = [[pt[i * D:(i + 1) * D] for i in range(nv)] for pt in points]
+    pts = [WP.Covector(c) for c in covectors] + [[pt[i * D:(i + 1) * D] for i in range(nv)] for pt in points]
     evs = [np.asarray(e, dtype=np.uint64) for e in evals]
     t0 = time.perf_counter()
     t = WP.Transcript(pid, sid)
