@@ -297,7 +297,10 @@ __device__ __forceinline__ int brev_small(int m) {
   return ((m & 1) << 3) | ((m & 2) << 1) | ((m & 4) >> 1) | (m >> 3);
 }
 
-template <int D, bool PAD, int R, int LOGR>
+// One radix-R DIF level over every sub-block of size S' = 2^logSub (cols columns of
+// S = 2^logS in LDS), outputs multiplied by w_S'^(base * bitrev(m)).  TW = false is
+// the last level (S' = R), where every such factor is w^0 = 1.
+template <int D, bool PAD, int R, int LOGR, bool TW>
 __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS, int logSub, int cols) {
   const int S = 1 << logS;
   const int stride = 1 << (logSub - LOGR);                 // S'/R
@@ -315,10 +318,19 @@ __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS
 #pragma unroll
     for (int m = 0; m < R; m++) {
       const int e = (base * brev_small<R>(m)) << tw_shift;
-      s[lidx<PAD>(e0 + m * stride)] = m == 0 ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
+      s[lidx<PAD>(e0 + m * stride)] = (m == 0 || !TW) ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
     }
   }
   (void)S;
+  (void)tw_shift;
+}
+
+template <int D, bool PAD, int R, int LOGR>
+__device__ __forceinline__ void dif_level(Ext<D>* s, const uint64_t* tw, int logS, int logSub, int cols) {
+  if (logSub == LOGR)
+    dif_step<D, PAD, R, LOGR, false>(s, tw, logS, logSub, cols);
+  else
+    dif_step<D, PAD, R, LOGR, true>(s, tw, logS, logSub, cols);
 }
 
 // R16: radix-16 steps first (4 stages per LDS round trip, 15/16 of a generic
@@ -328,21 +340,21 @@ __device__ __forceinline__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS,
   int logSub = logS;
   if (R16) {
     while (logSub >= 4) {
-      dif_step<D, PAD, 16, 4>(s, tw, logS, logSub, cols);
+      dif_level<D, PAD, 16, 4>(s, tw, logS, logSub, cols);
       __syncthreads();
       logSub -= 4;
     }
   }
   while (logSub >= 3) {
-    dif_step<D, PAD, 8, 3>(s, tw, logS, logSub, cols);
+    dif_level<D, PAD, 8, 3>(s, tw, logS, logSub, cols);
     __syncthreads();
     logSub -= 3;
   }
   if (logSub == 2) {
-    dif_step<D, PAD, 4, 2>(s, tw, logS, logSub, cols);
+    dif_step<D, PAD, 4, 2, false>(s, tw, logS, logSub, cols);
     __syncthreads();
   } else if (logSub == 1) {
-    dif_step<D, PAD, 2, 1>(s, tw, logS, logSub, cols);
+    dif_step<D, PAD, 2, 1, false>(s, tw, logS, logSub, cols);
     __syncthreads();
   }
 }
