@@ -32,6 +32,11 @@ HBM_PEAK_GBS = 8000.0
 # Per-launch HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from rocprofv3 --pmc
 # passes over this workload at d=24; matched to the dominant kernel by name.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_d24.json")
+# SQ_INSTS_VALU per launch of each kernel of the same d=24 command (and of the PoW
+# grinds), from a rocprofv3 --pmc pass: the instruction count is fixed per launch, so
+# instructions / the live launch time here is the achieved VALU issue rate.
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_d24.json")
+POW_KERNELS = {"blake3": "k_pow<4>", "sha3": "k_pow<3>"}
 
 
 def parse():
@@ -185,6 +190,14 @@ def main():
         roofline = {"kernel": dom_name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        if nv == 24 and os.path.exists(VALU_FILE) and avg_ms > 0:
+            vj = json.load(open(VALU_FILE))
+            entry = vj["kernels"].get(dom_name)
+            if entry:
+                rate = entry["valu_per_launch"] / (avg_ms * 1e-3) / 1e12
+                roofline["valu"] = {"instructions_per_launch": int(entry["valu_per_launch"]),
+                                    "achieved": round(rate, 4), "peak": round(vj["peak_tinstr_per_s"], 4),
+                                    "unit": "T wave-instr/s", "frac": round(rate / vj["peak_tinstr_per_s"], 4)}
 
     verified = None
     if not args.no_verify:
@@ -207,6 +220,18 @@ def main():
         cpu = cpu_baseline(nv, D, vec, points, evals, pid, sid, last.proof(),
                            [cov_host for _ in covs])
     pow_rates = None if args.no_pow else bench_pow(W, GP, nv)
+    if pow_rates and os.path.exists(VALU_FILE):
+        # Achieved VALU issue rate of the grinding kernels: their instructions per hash
+        # (profile: VALU per launch over the nonces of that launch) x the live hash rate.
+        vj = json.load(open(VALU_FILE))
+        for name, kern in POW_KERNELS.items():
+            entry = vj["kernels"].get(kern)
+            if entry:
+                per_hash = entry["valu_per_launch"] * entry["launches"] / vj["pow_nonces"]
+                rate = per_hash * pow_rates[f"{name}_ghashes_per_s"] * 1e9 / 1e12
+                pow_rates[f"{name}_valu"] = {"wave_instructions_per_hash": round(per_hash, 3),
+                                             "achieved": round(rate, 4), "peak": round(vj["peak_tinstr_per_s"], 4),
+                                             "unit": "T wave-instr/s", "frac": round(rate / vj["peak_tinstr_per_s"], 4)}
     others = None if args.no_other_configs else bench_other_configs(W, GP)
     batched = None if args.no_batch else bench_batched(args.workers)
     serving = None if args.no_serve else bench_serving(args.workers, args.serve_seconds, args.serve_qps)
