@@ -475,6 +475,15 @@ class DeviceBuffer:
             pass
 
 
+def memcpy_h2d(buf: DeviceBuffer, a) -> None:
+    """Copy a host array into the start of buf (synchronous)."""
+    a = np.ascontiguousarray(a)
+    if a.nbytes > buf.nbytes:
+        raise ValueError("memcpy_h2d: array larger than the buffer")
+    if a.nbytes:
+        _call("whir_memcpy_h2d", buf.ptr, a.ctypes.data, a.nbytes)
+
+
 def synchronize():
     _call("whir_synchronize")
 

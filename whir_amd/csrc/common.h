@@ -34,6 +34,11 @@ const char* last_error();
 // The stream the calling host thread launches on (created at default priority
 // on first use; one per thread so concurrent callers overlap on the device).
 hipStream_t thread_stream();
+// Frees this thread's stream (with its buffers and mailbox) and its pinned staging
+// and readback areas now; a worker thread calls it before it ends, so that no HIP
+// call runs in thread-exit destructors (where profiling tools intercepting HIP have
+// already torn down their per-thread state).  The thread may use HIP again after.
+void release_thread_resources();
 
 // Enqueue a host->device copy on stream s; `src` may be reused as soon as this
 // returns (small copies go through a pinned per-thread staging area).

@@ -153,7 +153,7 @@ struct whir_executor {
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || (j = pick()) != nullptr; });
-        if (!j) return;  // stop requested and nothing admissible left
+        if (!j) break;  // stop requested and nothing admissible left
         in_use += j->mem;
         running++;
         j->t_start = now_s(t0);
@@ -169,6 +169,7 @@ struct whir_executor {
       cv_done.notify_all();
       cv_work.notify_all();
     }
+    release_thread_resources();
   }
 };
 

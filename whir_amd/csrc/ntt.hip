@@ -610,7 +610,10 @@ static int floor_log2(int x) {
 // The defaults are the fastest measured on MI355X; WHIR_NTT_VARIANT = "pad,r16"
 // masks overrides them for tuning.
 struct NttVariant {
-  int pad = 0xC, r16 = 0x2;  // radix 16 pays only for D = 1 (D = 2 loses occupancy to registers)
+  // Radix 16 is off by default: with 512-thread blocks a 4096-word tile has fewer
+  // 16-point groups than threads (the autotuner measured radix 8 3% faster for Fp at
+  // 2^26 outputs), and for Fp2/Fp3 it also costs occupancy in registers.
+  int pad = 0xC, r16 = 0x0;
 };
 static const NttVariant& ntt_variant() {
   static NttVariant v = [] {
@@ -635,9 +638,8 @@ static NttSched default_sched(int D, int logC) {
   NttSched s;
   s.logC1 = logC <= MAX_PASS_LOG ? 0 : (logC + 1) / 2;
   s.v1 = s.v2 = variant_bits(D);
-  // Fp pass 1 (post factors in registers) is faster padded with radix-8 steps than
-  // with radix 16 (1.13 vs 1.22 ms for both passes at 2^26 outputs on MI355X, as
-  // measured by the autotuner); pass 2 keeps radix 16.
+  // Fp pass 1 (post factors in registers) is faster padded (1.13 vs 1.22 ms for both
+  // passes at 2^26 outputs on MI355X, as measured by the autotuner); pass 2 is not.
   if (D == 1 && !getenv("WHIR_NTT_VARIANT") && logC > MAX_PASS_LOG) s.v1 = 2;
   s.tw1 = s.tw2 = tile_words();
   return s;

@@ -22,7 +22,8 @@ namespace {
 struct ThreadStream {
   hipStream_t s = nullptr;
   int device = -1;
-  ~ThreadStream();
+  void release();
+  ~ThreadStream() { release(); }
 };
 thread_local ThreadStream t_stream;
 }  // namespace
@@ -53,10 +54,15 @@ struct Staging {
   size_t cap = 0, off = 0;
   hipEvent_t wrap = nullptr;
   int device = -1;
-  ~Staging() {
+  void release() {
     if (host) (void)hipHostFree(host);
     if (wrap) (void)hipEventDestroy(wrap);
+    host = nullptr;
+    wrap = nullptr;
+    cap = off = 0;
+    device = -1;
   }
+  ~Staging() { release(); }
 };
 thread_local Staging t_staging;
 }  // namespace
@@ -108,12 +114,14 @@ void pool_trim(hipStream_t s) {
 
 static void release_stream_slots(hipStream_t s);
 
-ThreadStream::~ThreadStream() {
+void ThreadStream::release() {
   if (s) {
     pool_trim(s);
     (void)hipStreamSynchronize(s);
     release_stream_slots(s);
     (void)hipStreamDestroy(s);
+    s = nullptr;
+    device = -1;
   }
 }
 
@@ -163,12 +171,22 @@ struct Readback {
   uint8_t* host = nullptr;
   size_t cap = 0;
   int device = -1;
-  ~Readback() {
+  void release() {
     if (host) (void)hipHostFree(host);
+    host = nullptr;
+    cap = 0;
+    device = -1;
   }
+  ~Readback() { release(); }
 };
 thread_local Readback t_readback;
 }  // namespace
+
+void release_thread_resources() {
+  t_stream.release();
+  t_staging.release();
+  t_readback.release();
+}
 
 void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (!bytes) {
