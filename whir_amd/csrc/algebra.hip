@@ -381,21 +381,35 @@ struct CovPlan {
   size_t words, entries;
 };
 
+// Inputs of the covector tables (eq points and scalars, power points and scalars) up
+// to this many words travel with the launch itself, which saves a staged copy and
+// its launch; larger inputs are copied into the table buffer first.
+static constexpr size_t COV_ARG_WORDS = 320;
+struct CovIn {
+  uint64_t w[COV_ARG_WORDS];
+};
+
+// Element i (D words) of the input region starting at word `off`: from device memory
+// when dev is given, else from the launch's arguments.
 template <int D>
-__global__ void k_cov_tables(uint64_t* buf, CovPlan L) {
+__device__ __forceinline__ Ext<D> in_ld(const uint64_t* dev, const CovIn& arg, size_t off, size_t i) {
+  Ext<D> r;
+#pragma unroll
+  for (int d = 0; d < D; d++) r.c[d] = dev ? dev[off + i * D + d] : arg.w[off + i * D + d];
+  return r;
+}
+
+template <int D>
+__global__ void k_cov_tables(uint64_t* buf, CovPlan L, const uint64_t* in_dev, CovIn in_arg) {
   size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e >= L.entries) return;
-  const uint64_t* z = buf + L.in_z;
-  const uint64_t* sc = buf + L.in_s;
-  const uint64_t* xs = buf + L.in_x;
-  const uint64_t* ts = buf + L.in_t;
   const size_t m = L.m_e + L.m_b;
   const size_t sec[7] = {L.np * L.nhi, L.np * L.nlo, m * L.nb, L.m_e * RT, L.m_e, L.m_b * RT, L.m_b};
   if (e < sec[0]) {
     const size_t p = e / L.nhi, h = e % L.nhi;
-    Ext<D> v = ld<D>(sc, p);
+    Ext<D> v = in_ld<D>(in_dev, in_arg, L.in_s, p);
     for (size_t q = 0; q < L.k1; q++) {
-      Ext<D> zq = ld<D>(z, p * L.nv + q);
+      Ext<D> zq = in_ld<D>(in_dev, in_arg, L.in_z, p * L.nv + q);
       v = gl::emul<D>(v, ((h >> (L.k1 - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
     }
     st<D>(buf + L.hi, e, v);
@@ -406,7 +420,7 @@ __global__ void k_cov_tables(uint64_t* buf, CovPlan L) {
     const size_t p = e / L.nlo, l = e % L.nlo;
     Ext<D> v = gl::ext_base<D>(1);
     for (size_t q = 0; q < L.k2; q++) {
-      Ext<D> zq = ld<D>(z, p * L.nv + L.k1 + q);
+      Ext<D> zq = in_ld<D>(in_dev, in_arg, L.in_z, p * L.nv + L.k1 + q);
       v = gl::emul<D>(v, ((l >> (L.k2 - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
     }
     st<D>(buf + L.lo, e, v);
@@ -416,28 +430,28 @@ __global__ void k_cov_tables(uint64_t* buf, CovPlan L) {
   if (e < sec[2]) {
     const size_t j = e / L.nb, b = e % L.nb;
     const uint64_t ex = (uint64_t)b * L.S;
-    Ext<D> v = j < L.m_e ? gl::emul<D>(ld<D>(ts, j), gl::epow<D>(ld<D>(xs, j), ex))
-                         : gl::escale<D>(ld<D>(ts, j), gl::pow(xs[j * D], ex));
+    const Ext<D> t = in_ld<D>(in_dev, in_arg, L.in_t, j), x = in_ld<D>(in_dev, in_arg, L.in_x, j);
+    Ext<D> v = j < L.m_e ? gl::emul<D>(t, gl::epow<D>(x, ex)) : gl::escale<D>(t, gl::pow(x.c[0], ex));
     st<D>(buf + L.T0, e, v);
     return;
   }
   e -= sec[2];
   if (e < sec[3]) {
-    st<D>(buf + L.T1e, e, gl::epow<D>(ld<D>(xs, e / RT), e % RT));
+    st<D>(buf + L.T1e, e, gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, e / RT), e % RT));
     return;
   }
   e -= sec[3];
   if (e < sec[4]) {
-    st<D>(buf + L.Xe, e, gl::epow<D>(ld<D>(xs, e), RT));
+    st<D>(buf + L.Xe, e, gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, e), RT));
     return;
   }
   e -= sec[4];
   if (e < sec[5]) {
-    buf[L.T1b + e] = gl::pow(xs[(L.m_e + e / RT) * D], e % RT);
+    buf[L.T1b + e] = gl::pow(in_ld<D>(in_dev, in_arg, L.in_x, L.m_e + e / RT).c[0], e % RT);
     return;
   }
   e -= sec[5];
-  buf[L.Xb + e] = gl::pow(xs[(L.m_e + e) * D], RT);
+  buf[L.Xb + e] = gl::pow(in_ld<D>(in_dev, in_arg, L.in_x, L.m_e + e).c[0], RT);
 }
 
 template <int D, int K>
@@ -532,7 +546,9 @@ static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t*
   L.in_t = in.size();
   for (size_t j : order_e) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
   for (size_t j : order_b) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
-  size_t w = in.size();
+  // Small inputs go with the launch; larger ones are copied to the front of the buffer.
+  const bool by_arg = in.size() <= COV_ARG_WORDS;
+  size_t w = by_arg ? 0 : in.size();
   L.hi = w;  w += np * L.nhi * D;
   L.lo = w;  w += np * L.nlo * D;
   L.T0 = w;  w += m * L.nb * D;
@@ -543,8 +559,11 @@ static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t*
   L.words = w;
   L.entries = np * (L.nhi + L.nlo) + m * L.nb + L.m_e * (RT + 1) + L.m_b * (RT + 1);
   DevBuf buf(L.words * 8, s);
-  if (!in.empty()) upload_async(buf.ptr, in.data(), in.size() * 8, s);
-  k_cov_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
+  CovIn args{};
+  if (by_arg) std::copy(in.begin(), in.end(), args.w);
+  else upload_async(buf.ptr, in.data(), in.size() * 8, s);
+  k_cov_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L, by_arg ? nullptr : buf.as<uint64_t>(),
+                                                            args);
   WHIR_HIP(hipGetLastError());
   ProfScope prof(kernel_name("k_cov", D, K), s, 8.0 * D * (accumulate ? 2.0 : 1.0) * (double)n);
   const unsigned g = (unsigned)L.nb;
@@ -591,19 +610,23 @@ static constexpr int UNI_MP = 4;
 
 struct UniPlan {
   size_t m, nb, S;
-  size_t in_x, P, B, words, entries;
+  size_t P, B, words, entries;
+  uint64_t x[UNI_MP * 3];  // the points (m x D words), passed with the launch
 };
 
 template <int D>
 __global__ void k_uni_tables(uint64_t* buf, UniPlan L) {
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e >= L.entries) return;
-  const uint64_t* xs = buf + L.in_x;
+  const size_t j = e < L.m * L.S ? e / L.S : (e - L.m * L.S) / L.nb;
+  Ext<D> x;
+#pragma unroll
+  for (int d = 0; d < D; d++) x.c[d] = L.x[j * D + d];
   if (e < L.m * L.S) {
-    st<D>(buf + L.P, e, gl::epow<D>(ld<D>(xs, e / L.S), e % L.S));
+    st<D>(buf + L.P, e, gl::epow<D>(x, e % L.S));
   } else {
     const size_t f = e - L.m * L.S;
-    st<D>(buf + L.B, f, gl::epow<D>(ld<D>(xs, f / L.nb), (uint64_t)(f % L.nb) * L.S));
+    st<D>(buf + L.B, f, gl::epow<D>(x, (uint64_t)(f % L.nb) * L.S));
   }
 }
 
@@ -649,13 +672,12 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     L.m = mj;
     L.S = (size_t)RT * UNI_K;
     L.nb = (n + L.S - 1) / L.S;
-    L.in_x = 0;
-    L.P = mj * D;
+    std::copy(xs + j0 * D, xs + (j0 + mj) * D, L.x);
+    L.P = 0;
     L.B = L.P + mj * L.S * D;
     L.words = L.B + mj * L.nb * D;
     L.entries = mj * (L.S + L.nb);
     DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s), res(UNI_MP * D * 8, s);
-    upload_async(buf.ptr, xs + j0 * D, mj * D * 8, s);
     k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
     WHIR_HIP(hipGetLastError());
     // The reduction writes all UNI_MP slots; only the first mj go to the caller
