@@ -117,6 +117,15 @@ def test_merkle_commit_matches_oracle(engine, n):
     assert W.merkle_commit(engine, leaves) == O.merkle_commit(engine, leaves)
 
 
+@pytest.mark.parametrize("n", [1 << 19, (1 << 20) + 3])
+def test_merkle_commit_wide_levels_match_oracle(n):
+    """Trees wide enough for the eight-nodes-per-lane first phase of the subtree
+    kernel (levels of 2^19 nodes and more), then its pair levels, for BLAKE3."""
+    rng = np.random.default_rng(n)
+    leaves = rng.integers(0, 256, size=32 * n, dtype=np.uint8).tobytes()
+    assert W.merkle_commit(W.HASH_BLAKE3, leaves) == O.merkle_commit(W.HASH_BLAKE3, leaves)
+
+
 @pytest.mark.parametrize("D", [1, 2, 3])
 @pytest.mark.parametrize("engine", [W.HASH_BLAKE3, W.HASH_SHA3])
 def test_matrix_commit_and_column_leaves(D, engine):
