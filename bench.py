@@ -52,7 +52,10 @@ def parse():
                     "Goldilocks3 prover timings")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pow", action="store_true", help="skip the PoW grinding-rate measurement")
-    ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams)")
+    ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams) for serving")
+    ap.add_argument("--batch-workers", type=int, default=12,
+                    help="executor worker threads for the batched measurement (12 measured ~10%% faster than "
+                         "8 for the 64-instance batch on MI355X; serving keeps 8, which gave the lower p99)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="arrival rate; 0 = 70%% of measured capacity")
     ap.add_argument("--inputs", choices=("uniform", "cli"), default="uniform",
@@ -244,7 +247,7 @@ def main():
                                              "achieved": round(rate, 4), "peak": round(vj["peak_tinstr_per_s"], 4),
                                              "unit": "T wave-instr/s", "frac": round(rate / vj["peak_tinstr_per_s"], 4)}
     others = None if args.no_other_configs else bench_other_configs(W, GP)
-    batched = None if args.no_batch else bench_batched(args.workers)
+    batched = None if args.no_batch else bench_batched(args.batch_workers)
     serving = None if args.no_serve else bench_serving(args.workers, args.serve_seconds, args.serve_qps)
 
     kernels = {k: {"launches": v["launches"], "ms_per_step": round(v["total_ms"] / args.steps, 4)}
