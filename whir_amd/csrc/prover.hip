@@ -204,8 +204,32 @@ void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::ve
   const size_t N = c.codeword_length;
   uint64_t C = coset_size(c.masked_message_length(), N), Q = N / C;
   uint64_t g = gl::root_of_unity(N);
+  // g^e for every index (e < N, transpose_permute order) from a table of g^(d 16^w),
+  // d < 16, one row per 4-bit window of e: 15 multiplies per window to build and one
+  // per window per point, against a square-and-multiply per point (these run on the
+  // host between two GPU steps of the proof).
+  int bits = 0;
+  while (((uint64_t)1 << bits) < N) bits++;
+  const int windows = std::max(1, (bits + 3) / 4);
+  std::vector<uint64_t> tab((size_t)windows * 16);
+  uint64_t base = g;
+  for (int w = 0; w < windows; w++) {
+    uint64_t* row = &tab[(size_t)w * 16];
+    row[0] = 1;
+    row[1] = base;
+    for (int d = 2; d < 16; d++) row[d] = gl::mul(row[d - 1], base);
+    base = gl::mul(row[15], base);
+  }
   pts.clear();
-  for (size_t i : idx) pts.push_back(gl::pow(g, (i / C) + (i % C) * Q));  // transpose_permute
+  for (size_t i : idx) {
+    const uint64_t e = (i / C) + (i % C) * Q;
+    uint64_t v = 1;
+    for (int w = 0; w < windows; w++) {
+      const unsigned d = (unsigned)(e >> (4 * w)) & 15;
+      if (d) v = v == 1 ? tab[(size_t)w * 16 + d] : gl::mul(v, tab[(size_t)w * 16 + d]);
+    }
+    pts.push_back(v);
+  }
 }
 
 // The per-witness part of irs_commit::Config::open (irs_commit.rs:410-428): hints
@@ -262,7 +286,7 @@ void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& id
     }
   }
   t.hint_field_vec(w.Ds, rows.data(), k * M);
-  for (size_t i = 0; i < ns; i++) t.hint_hash(&hashes[32 * i]);
+  t.hint_hashes(hashes.data(), ns);
 }
 
 // irs_commit::Config::open (irs_commit.rs:385-432) over one or more witnesses:
@@ -487,10 +511,10 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
       for (size_t col = 0; col < M; col++) {
         const size_t g = w * M + col;  // column of the concatenated rows = vector * depth + chunk
         E<D> wt = gl::emul<D>(poly_rlc[g / depth], w_eq[g % depth]);
-        for (size_t q = 0; q < k; q++) {
-          const uint64_t* e = &rows[w][(q * M + col) * Ds];
-          E<D> v = Ds == 1 ? gl::ext_base<D>(e[0]) : ld<D>(e);
-          out[q] = gl::eadd<D>(out[q], gl::emul<D>(wt, v));
+        if (Ds == 1) {
+          for (size_t q = 0; q < k; q++) out[q] = gl::eadd<D>(out[q], gl::escale<D>(wt, rows[w][q * M + col]));
+        } else {
+          for (size_t q = 0; q < k; q++) out[q] = gl::eadd<D>(out[q], gl::emul<D>(wt, ld<D>(&rows[w][(q * M + col) * Ds])));
         }
       }
     }

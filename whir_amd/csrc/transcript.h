@@ -26,6 +26,10 @@ struct Transcript {
   virtual void prover_hash(const uint8_t* h32) = 0;
   virtual void prover_u64(uint64_t v) = 0;
   virtual void hint_hash(const uint8_t* h32) = 0;
+  // n hashes of 32 bytes, one hint_hash each
+  virtual void hint_hashes(const uint8_t* h, size_t n) {
+    for (size_t i = 0; i < n; i++) hint_hash(h + 32 * i);
+  }
   // prover_hint_ark(&Vec<F>)
   virtual void hint_field_vec(int degree, const uint64_t* elems, size_t count) = 0;
 };
@@ -42,6 +46,7 @@ class Duplex final : public Transcript {
   void prover_hash(const uint8_t* h32) override;
   void prover_u64(uint64_t v) override;
   void hint_hash(const uint8_t* h32) override;
+  void hint_hashes(const uint8_t* h, size_t n) override;
   void hint_field_vec(int degree, const uint64_t* elems, size_t count) override;
 
   const std::vector<uint8_t>& narg() const { return narg_; }
