@@ -74,9 +74,12 @@ void dot_dev(int D, const uint64_t* a, const uint64_t* b, size_t n, uint64_t* de
 // out[k] = v[idx[k]] rows of a column-major matrix (gather for openings):
 // out[k*M + j] = cw[j*N + idx[k]] (degree-D elements).
 // Rows and Merkle sibling hashes of an opening in one launch: idx_sib = k row
-// indices then ns node indices (device); out_hashes 16-byte aligned.
+// indices then ns node indices (device); out_hashes 16-byte aligned.  D > 0 also
+// writes out_vals[q] = sum_j wts[j] row_q[j] (D words each; wts: M weights of D words
+// in device memory; rows of Ds words embedded in the degree-D extension).
 void gather_open_dev(int Ds, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
-                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t st);
+                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t st,
+                     int D = 0, const uint64_t* wts = nullptr, uint64_t* out_vals = nullptr);
 
 void gather_rows_dev(int D, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx, size_t k, uint64_t* out,
                      hipStream_t st);

@@ -807,11 +807,13 @@ __global__ void k_gather(const uint64_t* cw, size_t N, size_t M, const uint64_t*
 
 // Opening gather in one launch: rows (out_rows[q*M + j] = cw[j*N + idx[q]], Ds words
 // each) and Merkle sibling hashes (out_hashes[i] = nodes[sib[i]]); idx_sib holds the
-// k row indices followed by the ns node indices.
-template <int Ds>
+// k row indices followed by the ns node indices.  With D > 0 it also sums each opened
+// row against per-column weights of degree D (wts[j], D words):
+// out_vals[q] = sum_j wts[j] cw[j*N + idx[q]].
+template <int Ds, int D>
 __global__ void k_gather_open(const uint64_t* __restrict__ cw, size_t N, size_t M, const uint64_t* __restrict__ idx_sib,
                               size_t k, const uint4* __restrict__ nodes, size_t ns, uint64_t* out_rows,
-                              uint4* out_hashes) {
+                              uint4* out_hashes, const uint64_t* __restrict__ wts, uint64_t* out_vals) {
   size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e < k * M) {
     const size_t q = e / M, j = e % M, r = idx_sib[q];
@@ -819,22 +821,54 @@ __global__ void k_gather_open(const uint64_t* __restrict__ cw, size_t N, size_t 
     return;
   }
   e -= k * M;
-  if (e < 2 * ns) out_hashes[e] = nodes[2 * idx_sib[k + (e >> 1)] + (e & 1)];
+  if (e < 2 * ns) {
+    out_hashes[e] = nodes[2 * idx_sib[k + (e >> 1)] + (e & 1)];
+    return;
+  }
+  if constexpr (D > 0) {
+    e -= 2 * ns;
+    if (e < k) {
+      const size_t r = idx_sib[e];
+      Ext<D> acc = gl::ext_zero<D>();
+      if (r < N) {
+        for (size_t j = 0; j < M; j++) {
+          if constexpr (Ds == 1) acc = gl::eadd<D>(acc, gl::escale<D>(ld<D>(wts, j), cw[j * N + r]));
+          else acc = gl::eadd<D>(acc, gl::emul<D>(ld<D>(wts, j), ld<D>(cw, j * N + r)));
+        }
+      }
+      st<D>(out_vals, e, acc);
+    }
+  }
+}
+
+template <int Ds, int D>
+static void gather_open_t(const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
+                          const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes,
+                          const uint64_t* wts, uint64_t* out_vals, hipStream_t s) {
+  const size_t total = k * M + 2 * ns + (D > 0 ? k : 0);
+  if (!total) return;
+  k_gather_open<Ds, D><<<blocks_for(total, RT), RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows,
+                                                              (uint4*)out_hashes, wts, out_vals);
+  WHIR_HIP(hipGetLastError());
 }
 
 void gather_open_dev(int Ds, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
-                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t s) {
-  const size_t total = k * M + 2 * ns;
-  if (!total) return;
+                     const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes, hipStream_t s, int D,
+                     const uint64_t* wts, uint64_t* out_vals) {
   WHIR_REQUIRE(ns == 0 || ((uintptr_t)out_hashes & 15) == 0, "gather_open: hash output must be 16-byte aligned");
-  const unsigned g = blocks_for(total, RT);
-  switch (Ds) {
-    case 1: k_gather_open<1><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
-    case 2: k_gather_open<2><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
-    case 3: k_gather_open<3><<<g, RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows, (uint4*)out_hashes); break;
-    default: throw Error("gather: extension degree must be 1, 2 or 3");
+  WHIR_REQUIRE(D == 0 || (wts && out_vals), "gather_open: row sums need weights and an output");
+  const int key = Ds * 10 + D;
+  switch (key) {
+    case 10: gather_open_t<1, 0>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 20: gather_open_t<2, 0>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 30: gather_open_t<3, 0>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 11: gather_open_t<1, 1>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 12: gather_open_t<1, 2>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 13: gather_open_t<1, 3>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 22: gather_open_t<2, 2>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    case 33: gather_open_t<3, 3>(cw, N, M, idx_sib, k, nodes, ns, out_rows, out_hashes, wts, out_vals, s); return;
+    default: throw Error("gather: unsupported row / weight degrees");
   }
-  WHIR_HIP(hipGetLastError());
 }
 
 void gather_rows_dev(int D, const uint64_t* cw, size_t N, size_t M, const uint64_t* idx, size_t k, uint64_t* out,

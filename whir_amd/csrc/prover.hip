@@ -234,9 +234,10 @@ void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::ve
 
 // The per-witness part of irs_commit::Config::open (irs_commit.rs:410-428): hints
 // the rows at idx and their Merkle path (merkle_tree.rs:145-186); rows returned
-// (k x M elements of Ds words).
+// (k x M elements of Ds words).  With D > 0 the same launch also sums each row
+// against the M column weights wts (D words each) into vals (k x D words).
 void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& idx, std::vector<uint64_t>& rows,
-               hipStream_t s) {
+               hipStream_t s, int D = 0, const uint64_t* wts = nullptr, uint64_t* vals = nullptr) {
   const size_t N = w.N, M = w.M, k = idx.size();
   // Sibling node indices in hint order (they depend on the indices alone, so rows
   // and siblings are gathered in one launch).
@@ -263,27 +264,36 @@ void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& id
   }
   const size_t ns = idx_sib.size() - k;
   const size_t row_words = k * M * w.Ds, hash_off = (row_words + 1) & ~(size_t)1;  // hashes 16-byte aligned
+  const size_t val_off = hash_off + 4 * ns, out_words = val_off + (D > 0 ? k * D : 0);
   rows.assign(row_words, 0);
   std::vector<uint8_t> hashes(32 * ns);
+  if (D > 0) std::fill(vals, vals + k * D, 0);
   if (k + ns) {
+    // one upload: the row and node indices, then the column weights
+    const size_t nidx = idx_sib.size();
+    if (D > 0) idx_sib.insert(idx_sib.end(), wts, wts + M * D);
     DevBuf dis(idx_sib.size() * 8, s);
     upload_async(dis.ptr, idx_sib.data(), idx_sib.size() * 8, s);
+    const uint64_t* dw = D > 0 ? dis.as<uint64_t>() + nidx : nullptr;
     uint64_t* mbox = result_mailbox(s);
-    if (hash_off + 4 * ns <= RESULT_MAILBOX_WORDS) {
+    const uint64_t* res = mbox;
+    std::vector<uint64_t> h;
+    if (out_words <= RESULT_MAILBOX_WORDS) {
       gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns, mbox,
-                      (uint8_t*)(mbox + hash_off), s);
+                      (uint8_t*)(mbox + hash_off), s, D, dw, mbox + val_off);
       WHIR_HIP(hipStreamSynchronize(s));
-      std::copy(mbox, mbox + row_words, rows.begin());
-      memcpy(hashes.data(), mbox + hash_off, hashes.size());
     } else {
-      DevBuf dout((hash_off + 4 * ns) * 8, s);
+      DevBuf dout(out_words * 8, s);
       gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns,
-                      dout.as<uint64_t>(), (uint8_t*)(dout.as<uint64_t>() + hash_off), s);
-      std::vector<uint64_t> h((hash_off + 4 * ns));
+                      dout.as<uint64_t>(), (uint8_t*)(dout.as<uint64_t>() + hash_off), s, D, dw,
+                      dout.as<uint64_t>() + val_off);
+      h.resize(out_words);
       download_sync(h.data(), dout.ptr, h.size() * 8, s);
-      std::copy(h.begin(), h.begin() + row_words, rows.begin());
-      memcpy(hashes.data(), h.data() + hash_off, hashes.size());
+      res = h.data();
     }
+    std::copy(res, res + row_words, rows.begin());
+    memcpy(hashes.data(), res + hash_off, hashes.size());
+    if (D > 0) std::copy(res + val_off, res + val_off + k * D, vals);
   }
   t.hint_field_vec(w.Ds, rows.data(), k * M);
   t.hint_hashes(hashes.data(), ns);
@@ -291,13 +301,18 @@ void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& id
 
 // irs_commit::Config::open (irs_commit.rs:385-432) over one or more witnesses:
 // one set of in-domain indices, then each witness's rows and Merkle path.
-// rows[w] holds witness w's opened rows.
+// rows[w] holds witness w's opened rows; with D > 0, vals[w] (k x D words) their sums
+// against the column weights wts[w] (M x D words).
 void irs_open(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness*>& ws, std::vector<uint64_t>& pts,
-              std::vector<std::vector<uint64_t>>& rows, hipStream_t s) {
+              std::vector<std::vector<uint64_t>>& rows, hipStream_t s, int D = 0,
+              const std::vector<std::vector<uint64_t>>* wts = nullptr,
+              std::vector<std::vector<uint64_t>>* vals = nullptr) {
   std::vector<size_t> idx;
   in_domain(t, c, idx, pts);
   rows.resize(ws.size());
-  for (size_t w = 0; w < ws.size(); w++) open_rows(t, *ws[w], idx, rows[w], s);
+  if (D > 0) vals->assign(ws.size(), std::vector<uint64_t>(idx.size() * D));
+  for (size_t w = 0; w < ws.size(); w++)
+    open_rows(t, *ws[w], idx, rows[w], s, D, D > 0 ? (*wts)[w].data() : nullptr, D > 0 ? (*vals)[w].data() : nullptr);
 }
 
 // sumcheck::Config::prove (no masks), staggered fold + round polynomial.  The
@@ -501,24 +516,19 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
 
   // In-domain value of opened row q: dot(tensor_product(poly_rlc, eq_weights(r)),
   // row) over the rows of every opened witness side by side (irs_commit.rs:535-537).
-  auto row_values = [&](const std::vector<const DevWitness*>& pw, const std::vector<std::vector<uint64_t>>& rows,
-                        size_t k, const std::vector<E<D>>& poly_rlc, const std::vector<E<D>>& w_eq) {
-    std::vector<E<D>> out(k, gl::ext_zero<D>());
+  // The weight of column col of witness w (column w*M + col of the concatenated rows,
+  // i.e. vector g / depth, chunk g % depth); the gather launch of each opening sums
+  // its rows against them, and the witnesses' sums are added here.
+  auto column_weights = [&](const std::vector<const DevWitness*>& pw, const std::vector<E<D>>& poly_rlc,
+                            const std::vector<E<D>>& w_eq) {
+    std::vector<std::vector<uint64_t>> wts(pw.size());
     const size_t depth = w_eq.size();
-    for (size_t w = 0; w < pw.size(); w++) {
-      const size_t M = pw[w]->M;
-      const int Ds = pw[w]->Ds;
-      for (size_t col = 0; col < M; col++) {
-        const size_t g = w * M + col;  // column of the concatenated rows = vector * depth + chunk
-        E<D> wt = gl::emul<D>(poly_rlc[g / depth], w_eq[g % depth]);
-        if (Ds == 1) {
-          for (size_t q = 0; q < k; q++) out[q] = gl::eadd<D>(out[q], gl::escale<D>(wt, rows[w][q * M + col]));
-        } else {
-          for (size_t q = 0; q < k; q++) out[q] = gl::eadd<D>(out[q], gl::emul<D>(wt, ld<D>(&rows[w][(q * M + col) * Ds])));
-        }
+    for (size_t w = 0; w < pw.size(); w++)
+      for (size_t col = 0; col < pw[w]->M; col++) {
+        const size_t g = w * pw[w]->M + col;
+        push<D>(wts[w], gl::emul<D>(poly_rlc[g / depth], w_eq[g % depth]));
       }
-    }
-    return out;
+    return wts;
   };
 
   const IrsCfg* prev_cfg = &cfg.initial;
@@ -530,10 +540,13 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     irs_commit<D>(t, rc.irs, eng, &vp, 1, D, *nw, s);
     pow_prove(t, eng, rc.pow, s);
     std::vector<uint64_t> pts;
-    std::vector<std::vector<uint64_t>> rows;
-    irs_open(t, *prev_cfg, prev_ws, pts, rows, s);
+    std::vector<std::vector<uint64_t>> rows, vals;
+    const std::vector<std::vector<uint64_t>> wts = column_weights(prev_ws, vector_rlc, eq_weights<D>(fr));
+    irs_open(t, *prev_cfg, prev_ws, pts, rows, s, D, &wts, &vals);
     // STIR constraints: OOD points of the new commitment, then the opened rows.
-    std::vector<E<D>> svals = row_values(prev_ws, rows, pts.size(), vector_rlc, eq_weights<D>(fr));
+    std::vector<E<D>> svals(pts.size(), gl::ext_zero<D>());
+    for (const auto& v : vals)
+      for (size_t q = 0; q < pts.size(); q++) svals[q] = gl::eadd<D>(svals[q], ld<D>(&v[q * D]));
     std::vector<uint64_t> spoints(nw->ood_points);
     for (uint64_t x : pts) push<D>(spoints, gl::ext_base<D>(x));
     for (size_t k = nw->ood_values.size() / D; k-- > 0;) svals.insert(svals.begin(), ld<D>(&nw->ood_values[k * D]));
