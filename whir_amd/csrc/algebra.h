@@ -33,6 +33,15 @@ size_t fold_both_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint6
 size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n,
                                const uint64_t* w, uint64_t* dev_out, hipStream_t st);
 
+// Two sumcheck rounds per pass over vectors of length L (a power of two >= 4; a of Ds
+// words, b of D words).  mode 0: sums of round k and the coefficients of round k+1 in
+// r_k (8 values of D words: c0, c2, S0, S1, S2, T0, T1, T2) over (a_in, b); mode 1:
+// first folds (a_in, b) of length 4L by r1 then r2 into (a_out, b), then the same 8
+// sums over the folded vectors; mode 2: the fold, then (c0, c2) only; mode 3: the fold
+// alone.  r1, r2: host arrays of D words (unused in mode 0).
+void sumcheck_pair_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                       const uint64_t* r1, const uint64_t* r2, uint64_t* dev_out, hipStream_t st);
+
 // eval_eq (src/algebra/multilinear.rs:170-190): acc[x] += s * prod_i eq(z_i, x_i), x_0 = MSB.
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t st);
 

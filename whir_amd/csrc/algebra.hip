@@ -360,6 +360,157 @@ size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64
   return fold_and_poly_mixed_dev(D, D, a, a, b, n, w, out, s);
 }
 
+// ------------------------------------------------------------------ two sumcheck rounds per pass
+// Folding is linear in the challenge, so round k+1's polynomial is a polynomial in
+// round k's challenge r whose coefficients the pass over round k's vectors can sum
+// too.  For vectors x, y of length L = 4q (a power of two), h = L/2, dx_i = x_(h+i) -
+// x_i and, after the fold by r, x'_i = x_i + r dx_i:
+//   round k:    c0 = sum_(i<h) x_i y_i,  c2 = sum_(i<h) dx_i dy_i
+//   round k+1:  c0' = S0 + r S1 + r^2 S2,  c2' = T0 + r T1 + r^2 T2, with (i < q)
+//     S0 = sum x_i y_i,  S1 = sum (x_i dy_i + dx_i y_i),  S2 = sum dx_i dy_i,
+//     U = x_(q+i) - x_i,  V = dx_(q+i) - dx_i:  T0 = sum Ux Uy,  T1 = sum (Ux Vy + Vx Uy),
+//     T2 = sum Vx Vy
+// (the same field values as folding first and summing after, the order of exact
+// additions aside).  Lane i takes positions i, q+i, 2q+i, 3q+i.  With FOLD the vectors
+// are first folded by two pending challenges (r1, then r2) from length 4L, position t
+// from t, L+t, 2L+t, 3L+t of the source, and written (a to a_out, b in place: lane i
+// writes only its own four positions, the only ones below L that it reads).  NS: sums
+// per block, 8 (c0, c2, S0, S1, S2, T0, T1, T2), 2 (c0, c2) or 0.
+template <int Ds, int D>
+__device__ __forceinline__ Ext<D> fold_twice(const uint64_t* v, size_t t, size_t L, const Ext<D>& r1,
+                                             const Ext<D>& r2) {
+  const Ext<D> x0 = ldx<Ds, D>(v, t), x1 = ldx<Ds, D>(v, L + t), x2 = ldx<Ds, D>(v, 2 * L + t),
+               x3 = ldx<Ds, D>(v, 3 * L + t);
+  const Ext<D> p = gl::eadd<D>(x0, gl::emul<D>(gl::esub<D>(x2, x0), r1));
+  const Ext<D> q = gl::eadd<D>(x1, gl::emul<D>(gl::esub<D>(x3, x1), r1));
+  return gl::eadd<D>(p, gl::emul<D>(gl::esub<D>(q, p), r2));
+}
+
+template <int Ds, int D, bool FOLD, int NS>
+__global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                                                Ext<D> r1, Ext<D> r2, uint64_t* parts) {
+  constexpr int NA = NS > 0 ? NS : 1;
+  Ext<D> acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; k++) acc[k] = gl::ext_zero<D>();
+  const size_t q = L >> 2;
+  if constexpr (!FOLD && Ds == 1 && D > 1 && NS == 8) {
+    // x in the base field: every product is base x extension (D multiplies, not an
+    // extension product), and the differences of x are base-field differences.
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
+      uint64_t x[4];
+      Ext<D> y[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        x[k] = a_in[k * q + i];
+        y[k] = ld<D>(b, k * q + i);
+      }
+      const uint64_t dx0 = gl::sub(x[2], x[0]), dx1 = gl::sub(x[3], x[1]);
+      const Ext<D> dy0 = gl::esub<D>(y[2], y[0]), dy1 = gl::esub<D>(y[3], y[1]);
+      const Ext<D> m0 = gl::escale<D>(y[0], x[0]), d0 = gl::escale<D>(dy0, dx0);
+      acc[0] = gl::eadd<D>(acc[0], gl::eadd<D>(m0, gl::escale<D>(y[1], x[1])));
+      acc[1] = gl::eadd<D>(acc[1], gl::eadd<D>(d0, gl::escale<D>(dy1, dx1)));
+      acc[2] = gl::eadd<D>(acc[2], m0);
+      acc[3] = gl::eadd<D>(acc[3], gl::eadd<D>(gl::escale<D>(dy0, x[0]), gl::escale<D>(y[0], dx0)));
+      acc[4] = gl::eadd<D>(acc[4], d0);
+      const uint64_t ux = gl::sub(x[1], x[0]), vx = gl::sub(dx1, dx0);
+      const Ext<D> uy = gl::esub<D>(y[1], y[0]), vy = gl::esub<D>(dy1, dy0);
+      acc[5] = gl::eadd<D>(acc[5], gl::escale<D>(uy, ux));
+      acc[6] = gl::eadd<D>(acc[6], gl::eadd<D>(gl::escale<D>(vy, ux), gl::escale<D>(uy, vx)));
+      acc[7] = gl::eadd<D>(acc[7], gl::escale<D>(vy, vx));
+    }
+    block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D);
+    return;
+  }
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
+    Ext<D> x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const size_t t = k * q + i;
+      if constexpr (FOLD) {
+        x[k] = fold_twice<Ds, D>(a_in, t, L, r1, r2);
+        y[k] = fold_twice<D, D>(b, t, L, r1, r2);
+      } else {
+        x[k] = ldx<Ds, D>(a_in, t);
+        y[k] = ld<D>(b, t);
+      }
+    }
+    if constexpr (FOLD) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        st<D>(a_out, k * q + i, x[k]);
+        st<D>(b, k * q + i, y[k]);
+      }
+    }
+    if constexpr (NS > 0) {
+      const Ext<D> dx0 = gl::esub<D>(x[2], x[0]), dx1 = gl::esub<D>(x[3], x[1]);
+      const Ext<D> dy0 = gl::esub<D>(y[2], y[0]), dy1 = gl::esub<D>(y[3], y[1]);
+      const Ext<D> m0 = gl::emul<D>(x[0], y[0]), d0 = gl::emul<D>(dx0, dy0);
+      acc[0] = gl::eadd<D>(acc[0], gl::eadd<D>(m0, gl::emul<D>(x[1], y[1])));
+      acc[1] = gl::eadd<D>(acc[1], gl::eadd<D>(d0, gl::emul<D>(dx1, dy1)));
+      if constexpr (NS == 8) {
+        acc[2] = gl::eadd<D>(acc[2], m0);
+        acc[3] = gl::eadd<D>(acc[3], gl::eadd<D>(gl::emul<D>(x[0], dy0), gl::emul<D>(dx0, y[0])));
+        acc[4] = gl::eadd<D>(acc[4], d0);
+        const Ext<D> ux = gl::esub<D>(x[1], x[0]), vx = gl::esub<D>(dx1, dx0);
+        const Ext<D> uy = gl::esub<D>(y[1], y[0]), vy = gl::esub<D>(dy1, dy0);
+        acc[5] = gl::eadd<D>(acc[5], gl::emul<D>(ux, uy));
+        acc[6] = gl::eadd<D>(acc[6], gl::eadd<D>(gl::emul<D>(ux, vy), gl::emul<D>(vx, uy)));
+        acc[7] = gl::eadd<D>(acc[7], gl::emul<D>(vx, vy));
+      }
+    }
+  }
+  if constexpr (NS > 0) block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D);
+}
+
+template <int Ds, int D, bool FOLD, int NS>
+static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L, const uint64_t* r1,
+                      const uint64_t* r2, uint64_t* out, hipStream_t s) {
+  const size_t q = L >> 2;
+  const unsigned g = NS == 0 ? grid_for(q) : (q <= SMALL_PAIRS ? 1u : grid_for(q));
+  DevBuf parts(NS > 0 && g > 1 ? (size_t)g * NS * D * 8 : 0, s);
+  const Ext<D> w1 = r1 ? host_ext<D>(r1) : gl::ext_zero<D>(), w2 = r2 ? host_ext<D>(r2) : gl::ext_zero<D>();
+  {
+    ProfScope prof(kernel_name(FOLD ? (NS ? "k_sc_pair_fold" : "k_fold4") : "k_sc_pair", Ds, D), s,
+                   FOLD ? 8.0 * ((Ds + D) * 4.0 * L + 2.0 * D * L) : 8.0 * (Ds + D) * (double)L);
+    k_sc_pair<Ds, D, FOLD, NS><<<g, RT, 0, s>>>(a_in, a_out, b, L, w1, w2,
+                                                NS > 0 && g > 1 ? parts.as<uint64_t>() : out);
+    WHIR_HIP(hipGetLastError());
+  }
+  if constexpr (NS > 0) {
+    if (g > 1) finish_reduce<D, NS>(parts.as<uint64_t>(), g, out, s);
+  }
+}
+
+template <int Ds, int D>
+static void sc_pair_d(int mode, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L, const uint64_t* r1,
+                      const uint64_t* r2, uint64_t* out, hipStream_t s) {
+  switch (mode) {
+    case 0: sc_pair_t<Ds, D, false, 8>(a_in, a_out, b, L, r1, r2, out, s); return;
+    case 1: sc_pair_t<Ds, D, true, 8>(a_in, a_out, b, L, r1, r2, out, s); return;
+    case 2: sc_pair_t<Ds, D, true, 2>(a_in, a_out, b, L, r1, r2, out, s); return;
+    default: sc_pair_t<Ds, D, true, 0>(a_in, a_out, b, L, r1, r2, out, s); return;
+  }
+}
+
+void sumcheck_pair_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                       const uint64_t* r1, const uint64_t* r2, uint64_t* out, hipStream_t s) {
+  WHIR_REQUIRE(L >= 4 && (L & (L - 1)) == 0, "sumcheck pair: length must be a power of two >= 4");
+  if (Ds == D) {
+    switch (D) {
+      case 1: sc_pair_d<1, 1>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
+      case 2: sc_pair_d<2, 2>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
+      case 3: sc_pair_d<3, 3>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: sc_pair_d<1, 2>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
+      case 3: sc_pair_d<1, 3>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
+    }
+  }
+  throw Error("sumcheck pair: extension degree must be 1, 2 or 3");
+}
+
 // ------------------------------------------------------------------ covector terms
 // covector_dev adds (or writes) any mix of
 //   eq terms     s_p eq(z_p, i)      eval_eq (src/algebra/multilinear.rs:170-190)

@@ -320,10 +320,91 @@ void irs_open(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness
 // round over the committed base-field vector, which is folded into `a` without
 // being lifted first); from the first fold on it lives in `a`.  Each round's
 // (c0, c2) lands in the stream's pinned mailbox from the reducing launch itself.
+//
+// Rounds go two per pass over the vectors while they have at least 16 elements (a
+// power of two): the pass sums round k's polynomial and round k+1's coefficients in
+// round k's challenge (sumcheck_pair_dev), so round k+1's message needs no pass and no
+// host round trip of its own; the two challenges are folded into the vectors by the
+// next pass.  The messages, challenges and folded vectors are the ones the round-by-
+// round loop below makes.
+template <int D>
+std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src,
+                                       int a_Ds, uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s) {
+  std::vector<E<D>> chals, pend;  // pend: challenges not yet folded into (a_src, b)
+  uint64_t* mbox = result_mailbox(s);
+  size_t len = n;                 // length after the pending folds
+  size_t round = 0;
+  auto send = [&](const E<D>& c0, const E<D>& c2) {
+    const E<D> c1 = gl::esub<D>(gl::esub<D>(gl::esub<D>(sum, c0), c0), c2);
+    t.prover_field(D, c0.c, 1);
+    t.prover_field(D, c2.c, 1);
+    pow_prove(t, engine, sc.round_pow, s);
+    const E<D> r = verifier_field<D>(t);
+    chals.push_back(r);
+    sum = gl::eadd<D>(gl::emul<D>(gl::eadd<D>(gl::emul<D>(c2, r), c1), r), c0);
+    return r;
+  };
+  while (round < sc.num_rounds) {
+    if (sc.num_rounds - round >= 2 && len >= 16 && pend.size() != 1) {
+      // round k's polynomial and round k+1's coefficients, folding the pending pair first
+      if (pend.empty()) {
+        sumcheck_pair_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, mbox, s);
+      } else {
+        sumcheck_pair_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        a_src = a;
+        a_Ds = D;
+      }
+      WHIR_HIP(hipStreamSynchronize(s));
+      uint64_t h[8 * D];
+      memcpy(h, mbox, sizeof h);
+      const E<D> r1 = send(ld<D>(h), ld<D>(h + D));
+      const E<D> r1sq = gl::emul<D>(r1, r1);
+      const E<D> c0 = gl::eadd<D>(gl::eadd<D>(ld<D>(h + 2 * D), gl::emul<D>(r1, ld<D>(h + 3 * D))),
+                                  gl::emul<D>(r1sq, ld<D>(h + 4 * D)));
+      const E<D> c2 = gl::eadd<D>(gl::eadd<D>(ld<D>(h + 5 * D), gl::emul<D>(r1, ld<D>(h + 6 * D))),
+                                  gl::emul<D>(r1sq, ld<D>(h + 7 * D)));
+      const E<D> r2 = send(c0, c2);
+      pend = {r1, r2};
+      len >>= 2;
+      round += 2;
+    } else {
+      // one round: its polynomial after folding whatever is pending
+      if (pend.size() == 2) {
+        sumcheck_pair_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        a_src = a;
+        a_Ds = D;
+      } else if (pend.size() == 1) {
+        fold_and_poly_mixed_dev(a_Ds, D, a_src, a, b, 2 * len, pend[0].c, mbox, s);
+        a_src = a;
+        a_Ds = D;
+      } else {
+        sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, len, mbox, s);
+      }
+      WHIR_HIP(hipStreamSynchronize(s));
+      uint64_t h[2 * D];
+      memcpy(h, mbox, sizeof h);
+      pend = {send(ld<D>(h), ld<D>(h + D))};
+      len >>= 1;
+      round += 1;
+    }
+  }
+  if (pend.size() == 2) {
+    sumcheck_pair_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, nullptr, s);
+  } else if (pend.size() == 1) {
+    fold_both_dev(a_Ds, D, a_src, a, b, 2 * len, pend[0].c, s);
+  } else if (a_src != a) {
+    lift_dev(D, a_src, len, a, s);
+  }
+  n = len;
+  return chals;
+}
+
 template <int D>
 std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src, int a_Ds,
                                  uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s) {
   WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
+  if (n >= 16 && (n & (n - 1)) == 0 && sc.num_rounds >= 2)
+    return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s);
   std::vector<E<D>> chals;
   uint64_t* mbox = result_mailbox(s);
   bool have_prev = false;
