@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -232,12 +233,27 @@ void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::ve
   }
 }
 
-// The per-witness part of irs_commit::Config::open (irs_commit.rs:410-428): hints
-// the rows at idx and their Merkle path (merkle_tree.rs:145-186); rows returned
-// (k x M elements of Ds words).  With D > 0 the same launch also sums each row
-// against the M column weights wts (D words each) into vals (k x D words).
-void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& idx, std::vector<uint64_t>& rows,
-               hipStream_t s, int D = 0, const uint64_t* wts = nullptr, uint64_t* vals = nullptr) {
+// The per-witness part of irs_commit::Config::open (irs_commit.rs:410-428): the rows at
+// idx and their Merkle path (merkle_tree.rs:145-186), gathered by one launch, and with
+// D > 0 each row's sum against the M column weights wts (D words each).  The launch
+// and the read-back are split so that a caller can queue more work (the covector and
+// the next sumcheck pass) behind the gather and synchronize once: open_rows_launch
+// queues the gather with its results in the stream's mailbox from word `off` (in a
+// device buffer when they do not fit), open_rows_finish, after the caller has
+// synchronized the stream, copies them out and hints rows (k x M elements of Ds words)
+// and hashes.  Hints do not enter the sponge, so hinting after later challenges were
+// drawn leaves the transcript as it is.
+struct OpenLaunch {
+  const DevWitness* w = nullptr;
+  size_t k = 0, M = 0, ns = 0, row_words = 0, hash_off = 0, val_off = 0, out_words = 0, off = 0;
+  int D = 0;
+  bool in_mailbox = true;
+  std::shared_ptr<DevBuf> dout;
+};
+
+OpenLaunch open_rows_launch(const DevWitness& w, const std::vector<size_t>& idx, hipStream_t s, int D,
+                            const uint64_t* wts, size_t off) {
+  OpenLaunch L;
   const size_t N = w.N, M = w.M, k = idx.size();
   // Sibling node indices in hint order (they depend on the indices alone, so rows
   // and siblings are gathered in one launch).
@@ -245,7 +261,7 @@ void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& id
   std::sort(cur.begin(), cur.end());
   cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
   std::vector<uint64_t> idx_sib(idx.begin(), idx.end());
-  size_t layers = merkle_layers(N), off = 0, width = (size_t)1 << layers;
+  size_t layers = merkle_layers(N), noff = 0, width = (size_t)1 << layers;
   while (width > 1) {
     std::vector<size_t> next;
     for (size_t q = 0; q < cur.size();) {
@@ -253,66 +269,98 @@ void open_rows(Transcript& t, const DevWitness& w, const std::vector<size_t>& id
       if (q + 1 < cur.size() && cur[q + 1] == (a ^ 1)) {
         q += 2;
       } else {
-        idx_sib.push_back(off + (a ^ 1));
+        idx_sib.push_back(noff + (a ^ 1));
         q += 1;
       }
       next.push_back(a >> 1);
     }
     cur.swap(next);
-    off += width;
+    noff += width;
     width >>= 1;
   }
-  const size_t ns = idx_sib.size() - k;
-  const size_t row_words = k * M * w.Ds, hash_off = (row_words + 1) & ~(size_t)1;  // hashes 16-byte aligned
-  const size_t val_off = hash_off + 4 * ns, out_words = val_off + (D > 0 ? k * D : 0);
-  rows.assign(row_words, 0);
-  std::vector<uint8_t> hashes(32 * ns);
-  if (D > 0) std::fill(vals, vals + k * D, 0);
-  if (k + ns) {
-    // one upload: the row and node indices, then the column weights
-    const size_t nidx = idx_sib.size();
-    if (D > 0) idx_sib.insert(idx_sib.end(), wts, wts + M * D);
-    DevBuf dis(idx_sib.size() * 8, s);
-    upload_async(dis.ptr, idx_sib.data(), idx_sib.size() * 8, s);
-    const uint64_t* dw = D > 0 ? dis.as<uint64_t>() + nidx : nullptr;
-    uint64_t* mbox = result_mailbox(s);
-    const uint64_t* res = mbox;
-    std::vector<uint64_t> h;
-    if (out_words <= RESULT_MAILBOX_WORDS) {
-      gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns, mbox,
-                      (uint8_t*)(mbox + hash_off), s, D, dw, mbox + val_off);
-      WHIR_HIP(hipStreamSynchronize(s));
-    } else {
-      DevBuf dout(out_words * 8, s);
-      gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), ns,
-                      dout.as<uint64_t>(), (uint8_t*)(dout.as<uint64_t>() + hash_off), s, D, dw,
-                      dout.as<uint64_t>() + val_off);
-      h.resize(out_words);
-      download_sync(h.data(), dout.ptr, h.size() * 8, s);
-      res = h.data();
-    }
-    std::copy(res, res + row_words, rows.begin());
-    memcpy(hashes.data(), res + hash_off, hashes.size());
-    if (D > 0) std::copy(res + val_off, res + val_off + k * D, vals);
+  L.w = &w;
+  L.k = k;
+  L.M = M;
+  L.D = D;
+  L.ns = idx_sib.size() - k;
+  L.row_words = k * M * w.Ds;
+  L.hash_off = (L.row_words + 1) & ~(size_t)1;  // hashes 16-byte aligned
+  L.val_off = L.hash_off + 4 * L.ns;
+  L.out_words = L.val_off + (D > 0 ? k * D : 0);
+  L.off = (off + 1) & ~(size_t)1;
+  if (!(k + L.ns)) return L;
+  // one upload: the row and node indices, then the column weights
+  const size_t nidx = idx_sib.size();
+  if (D > 0) idx_sib.insert(idx_sib.end(), wts, wts + M * D);
+  DevBuf dis(idx_sib.size() * 8, s);
+  upload_async(dis.ptr, idx_sib.data(), idx_sib.size() * 8, s);
+  const uint64_t* dw = D > 0 ? dis.as<uint64_t>() + nidx : nullptr;
+  uint64_t* out;
+  if (L.off + L.out_words <= RESULT_MAILBOX_WORDS) {
+    out = result_mailbox(s) + L.off;
+  } else {
+    L.in_mailbox = false;
+    L.dout = std::make_shared<DevBuf>(L.out_words * 8, s);
+    out = L.dout->as<uint64_t>();
   }
-  t.hint_field_vec(w.Ds, rows.data(), k * M);
-  t.hint_hashes(hashes.data(), ns);
+  gather_open_dev(w.Ds, w.cw.as<uint64_t>(), N, M, dis.as<uint64_t>(), k, w.nodes.as<uint8_t>(), L.ns, out,
+                  (uint8_t*)(out + L.hash_off), s, D, dw, out + L.val_off);
+  return L;
 }
 
+void open_rows_finish(Transcript& t, OpenLaunch& L, std::vector<uint64_t>& rows, uint64_t* vals, hipStream_t s) {
+  rows.assign(L.row_words, 0);
+  std::vector<uint8_t> hashes(32 * L.ns);
+  if (L.D > 0) std::fill(vals, vals + L.k * L.D, 0);
+  if (L.k + L.ns) {
+    const uint64_t* res;
+    std::vector<uint64_t> h;
+    if (L.in_mailbox) {
+      res = result_mailbox(s) + L.off;
+    } else {
+      h.resize(L.out_words);
+      download_sync(h.data(), L.dout->ptr, h.size() * 8, s);
+      res = h.data();
+      L.dout.reset();
+    }
+    std::copy(res, res + L.row_words, rows.begin());
+    memcpy(hashes.data(), res + L.hash_off, hashes.size());
+    if (L.D > 0) std::copy(res + L.val_off, res + L.val_off + L.k * L.D, vals);
+  }
+  t.hint_field_vec(L.w->Ds, rows.data(), L.k * L.M);
+  t.hint_hashes(hashes.data(), L.ns);
+}
+
+// Mailbox words below this offset are left to the sumcheck passes.
+constexpr size_t OPEN_MAILBOX_OFF = 64;
+
 // irs_commit::Config::open (irs_commit.rs:385-432) over one or more witnesses:
-// one set of in-domain indices, then each witness's rows and Merkle path.
-// rows[w] holds witness w's opened rows; with D > 0, vals[w] (k x D words) their sums
-// against the column weights wts[w] (M x D words).
-void irs_open(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness*>& ws, std::vector<uint64_t>& pts,
-              std::vector<std::vector<uint64_t>>& rows, hipStream_t s, int D = 0,
-              const std::vector<std::vector<uint64_t>>* wts = nullptr,
-              std::vector<std::vector<uint64_t>>* vals = nullptr) {
+// one set of in-domain indices, then each witness's rows and Merkle path.  Launch:
+// draws the indices, computes their points and queues every witness's gather.
+std::vector<OpenLaunch> irs_open_launch(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness*>& ws,
+                                        std::vector<uint64_t>& pts, hipStream_t s, int D = 0,
+                                        const std::vector<std::vector<uint64_t>>* wts = nullptr) {
   std::vector<size_t> idx;
   in_domain(t, c, idx, pts);
-  rows.resize(ws.size());
-  if (D > 0) vals->assign(ws.size(), std::vector<uint64_t>(idx.size() * D));
-  for (size_t w = 0; w < ws.size(); w++)
-    open_rows(t, *ws[w], idx, rows[w], s, D, D > 0 ? (*wts)[w].data() : nullptr, D > 0 ? (*vals)[w].data() : nullptr);
+  std::vector<OpenLaunch> ls;
+  size_t off = OPEN_MAILBOX_OFF;
+  for (size_t w = 0; w < ws.size(); w++) {
+    ls.push_back(open_rows_launch(*ws[w], idx, s, D, D > 0 ? (*wts)[w].data() : nullptr, off));
+    if (ls.back().in_mailbox) off = ls.back().off + ls.back().out_words;
+  }
+  return ls;
+}
+
+// Finish (after the stream was synchronized): rows[w] holds witness w's opened rows;
+// with D > 0, vals[w] (k x D words) their sums against the column weights.
+void irs_open_finish(Transcript& t, std::vector<OpenLaunch>& ls, std::vector<std::vector<uint64_t>>& rows,
+                     std::vector<std::vector<uint64_t>>& vals, hipStream_t s) {
+  rows.resize(ls.size());
+  vals.assign(ls.size(), std::vector<uint64_t>());
+  for (size_t w = 0; w < ls.size(); w++) {
+    if (ls[w].D > 0) vals[w].resize(ls[w].k * ls[w].D);
+    open_rows_finish(t, ls[w], rows[w], ls[w].D > 0 ? vals[w].data() : nullptr, s);
+  }
 }
 
 // sumcheck::Config::prove (no masks), staggered fold + round polynomial.  The
@@ -329,7 +377,14 @@ void irs_open(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness
 // round loop below makes.
 template <int D>
 std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src,
-                                       int a_Ds, uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s) {
+                                       int a_Ds, uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
+                                       const std::function<void()>& after_first) {
+  bool first = true;
+  auto synced = [&] {
+    WHIR_HIP(hipStreamSynchronize(s));
+    if (first && after_first) after_first();
+    first = false;
+  };
   std::vector<E<D>> chals, pend;  // pend: challenges not yet folded into (a_src, b)
   uint64_t* mbox = result_mailbox(s);
   size_t len = n;                 // length after the pending folds
@@ -354,7 +409,7 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
         a_src = a;
         a_Ds = D;
       }
-      WHIR_HIP(hipStreamSynchronize(s));
+      synced();
       uint64_t h[8 * D];
       memcpy(h, mbox, sizeof h);
       const E<D> r1 = send(ld<D>(h), ld<D>(h + D));
@@ -380,7 +435,7 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
       } else {
         sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, len, mbox, s);
       }
-      WHIR_HIP(hipStreamSynchronize(s));
+      synced();
       uint64_t h[2 * D];
       memcpy(h, mbox, sizeof h);
       pend = {send(ld<D>(h), ld<D>(h + D))};
@@ -399,12 +454,18 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
   return chals;
 }
 
+//
+// after_first, when given, runs once after the first synchronization (before the first
+// message is sent), or at the end when there are no rounds: a caller that queued
+// work ahead of the sumcheck (an opening's gather) reads its results there.
 template <int D>
 std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src, int a_Ds,
-                                 uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s) {
+                                 uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
+                                 const std::function<void()>& after_first = {}) {
   WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
   if (n >= 16 && (n & (n - 1)) == 0 && sc.num_rounds >= 2)
-    return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s);
+    return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s, after_first);
+  bool first = true;
   std::vector<E<D>> chals;
   uint64_t* mbox = result_mailbox(s);
   bool have_prev = false;
@@ -418,6 +479,8 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
       sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, n, mbox, s);
     }
     WHIR_HIP(hipStreamSynchronize(s));
+    if (first && after_first) after_first();
+    first = false;
     uint64_t h[2 * D];
     memcpy(h, mbox, sizeof h);
     E<D> c0 = ld<D>(h), c2 = ld<D>(h + D);
@@ -435,6 +498,10 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     n = fold_both_dev(a_Ds, D, a_src, a, b, n, prev.c, s);
   } else if (a_src != a) {
     lift_dev(D, a_src, n, a, s);
+  }
+  if (first && after_first) {
+    WHIR_HIP(hipStreamSynchronize(s));
+    after_first();
   }
   return chals;
 }
@@ -620,27 +687,37 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     const uint64_t* vp = vec.as<uint64_t>();
     irs_commit<D>(t, rc.irs, eng, &vp, 1, D, *nw, s);
     pow_prove(t, eng, rc.pow, s);
+    // The opening's gather, the covector terms and the sumcheck's first pass are queued
+    // back to back with one synchronization: the STIR challenge depends on the
+    // transcript alone (hints do not enter it), and the in-domain values, which only
+    // the claimed sum needs, come back with the first pass's sums.
     std::vector<uint64_t> pts;
-    std::vector<std::vector<uint64_t>> rows, vals;
     const std::vector<std::vector<uint64_t>> wts = column_weights(prev_ws, vector_rlc, eq_weights<D>(fr));
-    irs_open(t, *prev_cfg, prev_ws, pts, rows, s, D, &wts, &vals);
+    std::vector<OpenLaunch> opens = irs_open_launch(t, *prev_cfg, prev_ws, pts, s, D, &wts);
     // STIR constraints: OOD points of the new commitment, then the opened rows.
-    std::vector<E<D>> svals(pts.size(), gl::ext_zero<D>());
-    for (const auto& v : vals)
-      for (size_t q = 0; q < pts.size(); q++) svals[q] = gl::eadd<D>(svals[q], ld<D>(&v[q * D]));
+    const size_t nood = nw->ood_values.size() / D, kq = pts.size();
     std::vector<uint64_t> spoints(nw->ood_points);
     for (uint64_t x : pts) push<D>(spoints, gl::ext_base<D>(x));
-    for (size_t k = nw->ood_values.size() / D; k-- > 0;) svals.insert(svals.begin(), ld<D>(&nw->ood_values[k * D]));
-    std::vector<E<D>> srlc = geometric_challenge<D>(t, svals.size());
+    std::vector<E<D>> srlc = geometric_challenge<D>(t, nood + kq);
     if (!srlc.empty()) {
       std::vector<uint64_t> flat;
       for (auto& e : srlc) push<D>(flat, e);
       covector_dev(D, cov.as<uint64_t>(), n, true, nullptr, nullptr, 0, 0, flat.data(), spoints.data(),
                    srlc.size(), s);
     }
-    for (size_t k = 0; k < srlc.size(); k++) the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], svals[k]));
+    for (size_t k = 0; k < nood; k++)
+      the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], ld<D>(&nw->ood_values[k * D])));
+    auto finish_open = [&] {
+      std::vector<std::vector<uint64_t>> rows, vals;
+      irs_open_finish(t, opens, rows, vals, s);
+      for (size_t q = 0; q < kq; q++) {
+        E<D> v = gl::ext_zero<D>();
+        for (const auto& vw : vals) v = gl::eadd<D>(v, ld<D>(&vw[q * D]));
+        the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[nood + q], v));
+      }
+    };
     fr = sumcheck_prove<D>(t, rc.sumcheck, eng, vec.as<uint64_t>(), D, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
-                           the_sum, s);
+                           the_sum, s, finish_open);
     eval_point.insert(eval_point.end(), fr.begin(), fr.end());
     held = std::move(nw);
     prev_cfg = &rc.irs;
@@ -653,13 +730,16 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
   download_sync(fin.data(), vec.ptr, n * D * 8, s);
   for (size_t i = 0; i < n; i++) t.prover_field(D, &fin[i * D], 1);
   pow_prove(t, eng, cfg.final_pow, s);
-  {
-    std::vector<uint64_t> pts;
-    std::vector<std::vector<uint64_t>> rows;
-    irs_open(t, *prev_cfg, prev_ws, pts, rows, s);
-  }
+  // The final opening is queued ahead of the final sumcheck's first pass and read back
+  // with it.
+  std::vector<uint64_t> fpts;
+  std::vector<OpenLaunch> fopens = irs_open_launch(t, *prev_cfg, prev_ws, fpts, s);
+  auto finish_final = [&] {
+    std::vector<std::vector<uint64_t>> rows, vals;
+    irs_open_finish(t, fopens, rows, vals, s);
+  };
   fr = sumcheck_prove<D>(t, cfg.final_sumcheck, eng, vec.as<uint64_t>(), D, vec.as<uint64_t>(), cov.as<uint64_t>(),
-                         n, the_sum, s);
+                         n, the_sum, s, finish_final);
   eval_point.insert(eval_point.end(), fr.begin(), fr.end());
   if (eval_point_out)
     for (size_t i = 0; i < eval_point.size(); i++)
