@@ -394,6 +394,16 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
 #pragma unroll
   for (int k = 0; k < NA; k++) acc[k] = gl::ext_zero<D>();
   const size_t q = L >> 2;
+  if constexpr (FOLD && NS == 0) {
+    // the double fold alone: one output position per lane (four times the lanes of the
+    // summing passes, which the short vectors at the end of a sumcheck need)
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < L; t += (size_t)gridDim.x * blockDim.x) {
+      const Ext<D> x = fold_twice<Ds, D>(a_in, t, L, r1, r2), y = fold_twice<D, D>(b, t, L, r1, r2);
+      st<D>(a_out, t, x);
+      st<D>(b, t, y);
+    }
+    return;
+  }
   if constexpr (!FOLD && Ds == 1 && D > 1 && NS == 8) {
     // x in the base field: every product is base x extension (D multiplies, not an
     // extension product), and the differences of x are base-field differences.
@@ -467,7 +477,7 @@ template <int Ds, int D, bool FOLD, int NS>
 static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L, const uint64_t* r1,
                       const uint64_t* r2, uint64_t* out, hipStream_t s) {
   const size_t q = L >> 2;
-  const unsigned g = NS == 0 ? grid_for(q) : (q <= SMALL_PAIRS ? 1u : grid_for(q));
+  const unsigned g = NS == 0 ? grid_for(L) : (q <= SMALL_PAIRS ? 1u : grid_for(q));
   DevBuf parts(NS > 0 && g > 1 ? (size_t)g * NS * D * 8 : 0, s);
   const Ext<D> w1 = r1 ? host_ext<D>(r1) : gl::ext_zero<D>(), w2 = r2 ? host_ext<D>(r2) : gl::ext_zero<D>();
   {
