@@ -775,20 +775,33 @@ struct UniPlan {
   uint64_t x[UNI_MP * 3];  // the points (m x D words), passed with the launch
 };
 
+// x_j^(2^b) for b < UNI_PW, computed on the host and passed with the launch: a table
+// entry x_j^e is then the product over the set bits of e, one multiply per set bit
+// instead of a squaring and a multiply per bit (the table launch runs one short wave
+// per SIMD, so its time is its instruction count).
+static constexpr int UNI_PW = 32;
+struct UniPow {
+  uint64_t w[UNI_MP * UNI_PW * 3];
+};
+
 template <int D>
-__global__ void k_uni_tables(uint64_t* buf, UniPlan L) {
+__global__ void k_uni_tables(uint64_t* buf, UniPlan L, UniPow pw) {
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e >= L.entries) return;
-  const size_t j = e < L.m * L.S ? e / L.S : (e - L.m * L.S) / L.nb;
-  Ext<D> x;
+  const bool in_p = e < L.m * L.S;
+  const size_t j = in_p ? e / L.S : (e - L.m * L.S) / L.nb;
+  uint64_t ex = in_p ? e % L.S : (uint64_t)((e - L.m * L.S) % L.nb) * L.S;
+  Ext<D> r = gl::ext_base<D>(1);
+  for (int b = 0; ex; b++, ex >>= 1) {
+    if (ex & 1) {
+      Ext<D> p;
 #pragma unroll
-  for (int d = 0; d < D; d++) x.c[d] = L.x[j * D + d];
-  if (e < L.m * L.S) {
-    st<D>(buf + L.P, e, gl::epow<D>(x, e % L.S));
-  } else {
-    const size_t f = e - L.m * L.S;
-    st<D>(buf + L.B, f, gl::epow<D>(x, (uint64_t)(f % L.nb) * L.S));
+      for (int d = 0; d < D; d++) p.c[d] = pw.w[(j * UNI_PW + b) * D + d];
+      r = gl::emul<D>(r, p);
+    }
   }
+  if (in_p) st<D>(buf + L.P, e, r);
+  else st<D>(buf + L.B, e - L.m * L.S, r);
 }
 
 template <int Ds, int D>
@@ -839,7 +852,17 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     L.words = L.B + mj * L.nb * D;
     L.entries = mj * (L.S + L.nb);
     DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s), res(UNI_MP * D * 8, s);
-    k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L);
+    WHIR_REQUIRE((uint64_t)(L.nb - 1) * L.S < ((uint64_t)1 << UNI_PW), "univariate: vector too long");
+    UniPow pw;
+    for (size_t j = 0; j < mj; j++) {
+      Ext<D> p;
+      for (int d = 0; d < D; d++) p.c[d] = xs[(j0 + j) * D + d];
+      for (int b = 0; b < UNI_PW; b++) {
+        for (int d = 0; d < D; d++) pw.w[(j * UNI_PW + b) * D + d] = p.c[d];
+        p = gl::emul<D>(p, p);
+      }
+    }
+    k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L, pw);
     WHIR_HIP(hipGetLastError());
     // The reduction writes all UNI_MP slots; only the first mj go to the caller
     // (straight into `out` when it has room for whole groups).
