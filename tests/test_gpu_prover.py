@@ -71,6 +71,10 @@ CASES = [
     (9, 2, W.HASH_BLAKE3, 1, 1, 1, 1),
     (14, 2, W.HASH_BLAKE3, 4, 4, 2, 2),
     (13, 3, W.HASH_SHA3, 2, 3, 1, 3),
+    # five rounds per sumcheck (two pairs and a single round after a pending pair) and a
+    # base-field run of paired rounds
+    (16, 2, W.HASH_BLAKE3, 5, 5, 1, 1),
+    (17, 1, W.HASH_BLAKE3, 4, 4, 1, 1),
 ]
 
 
