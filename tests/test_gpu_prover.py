@@ -78,6 +78,17 @@ CASES = [
 ]
 
 
+def test_gpu_proof_with_openings_through_device_buffers(monkeypatch):
+    """Openings too large for the pinned mailbox are gathered into a device buffer and
+    read back on their own (forced here by shrinking the mailbox room they may use)."""
+    monkeypatch.setenv("WHIR_OPEN_MAILBOX_WORDS", "16")
+    nv, D, engine, k0, k, lir, npts = 12, 2, W.HASH_BLAKE3, 4, 4, 1, 2
+    pid, sid, vec, points, params = make_case(nv, D, engine, k0, k, lir, npts)
+    _, _, narg, hints, _ = oracle_proof(nv, D, pid, sid, vec, points, params, True)
+    _, gnarg, ghints, _ = gpu_proof(nv, D, pid, sid, vec, points, params, True)
+    assert gnarg == narg and ghints == hints
+
+
 @pytest.mark.parametrize("nv,D,engine,k0,k,lir,npts", CASES)
 def test_gpu_proof_equals_oracle_proof(nv, D, engine, k0, k, lir, npts):
     pid, sid, vec, points, params = make_case(nv, D, engine, k0, k, lir, npts)

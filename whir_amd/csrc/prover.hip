@@ -296,7 +296,11 @@ OpenLaunch open_rows_launch(const DevWitness& w, const std::vector<size_t>& idx,
   upload_async(dis.ptr, idx_sib.data(), idx_sib.size() * 8, s);
   const uint64_t* dw = D > 0 ? dis.as<uint64_t>() + nidx : nullptr;
   uint64_t* out;
-  if (L.off + L.out_words <= RESULT_MAILBOX_WORDS) {
+  // WHIR_OPEN_MAILBOX_WORDS lowers the mailbox room openings may use (a test knob that
+  // sends them through the device-buffer path).
+  size_t room = RESULT_MAILBOX_WORDS;
+  if (const char* e = getenv("WHIR_OPEN_MAILBOX_WORDS")) room = std::min<size_t>(room, (size_t)atoll(e));
+  if (L.off + L.out_words <= room) {
     out = result_mailbox(s) + L.off;
   } else {
     L.in_mailbox = false;
