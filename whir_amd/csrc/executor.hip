@@ -87,13 +87,23 @@ struct whir_executor {
   std::vector<std::thread> workers;
   size_t budget = 0, in_use = 0;
   int running = 0;
+  int running_large = 0;  // running jobs of vectors of at least 2^large_log elements
+  // While a large job runs, at most large_cap jobs run: the large job keeps more of the
+  // device, which cuts the tail latency of a mixed request stream (serving at 750
+  // requests/s, d log-uniform [14, 24], 8 workers: p99 13.6 -> 12.6-13.1 ms, p50 +2-4%;
+  // vectors of 2^23 and up: the 2^16..2^22 batch keeps its throughput).
+  int large_log = 23, large_cap = 6;
   uint64_t next_id = 1;
   bool stop = false;
   int device = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
-  // First queued job that fits the remaining budget (or any job when idle).
+  bool is_large(const Job* j) const { return j->cfg->cfg.initial.vector_size >= ((size_t)1 << large_log); }
+
+  // First queued job that fits the remaining budget (or any job when idle).  With a
+  // large-job cap, no job starts while a large one runs beside large_cap - 1 others.
   Job* pick() {
+    if (large_cap > 0 && running_large > 0 && running >= large_cap) return nullptr;
     for (auto it = queue.begin(); it != queue.end(); ++it) {
       Job* j = *it;
       if (running == 0 || in_use + j->mem <= budget) {
@@ -156,6 +166,7 @@ struct whir_executor {
         if (!j) break;  // stop requested and nothing admissible left
         in_use += j->mem;
         running++;
+        if (is_large(j)) running_large++;
         j->t_start = now_s(t0);
       }
       run(j);
@@ -165,6 +176,7 @@ struct whir_executor {
         j->done = true;
         in_use -= j->mem;
         running--;
+        if (is_large(j)) running_large--;
       }
       cv_done.notify_all();
       cv_work.notify_all();
@@ -208,6 +220,10 @@ int whir_executor_new(int num_workers, size_t hbm_budget, whir_executor** out) {
     hbm_budget = free_b / 10 * 8;  // leave a fifth of what is free for tables and other users
   }
   e->budget = hbm_budget;
+  // Large-job cap: while a job of at least 2^WHIR_EXEC_LARGE_LOG elements runs, no job
+  // starts while WHIR_EXEC_LARGE_CAP jobs run (0 turns the cap off).
+  if (const char* v = getenv("WHIR_EXEC_LARGE_CAP")) e->large_cap = atoi(v);
+  if (const char* v = getenv("WHIR_EXEC_LARGE_LOG")) e->large_log = atoi(v);
   whir_executor* p = e.get();
   for (int i = 0; i < num_workers; i++) e->workers.emplace_back([p] { p->loop(); });
   *out = e.release();
