@@ -72,6 +72,37 @@ def test_executor_proofs_equal_plain(mix, workers):
         ex.close()
 
 
+def test_executor_large_job_cap(mix, monkeypatch):
+    """A job of 2^23 elements among small ones: while it runs the executor admits at
+    most the capped number of jobs (here 2, so the cap is what orders the mix); every
+    proof still equals its plain one and the executor drains."""
+    monkeypatch.setenv("WHIR_EXEC_LARGE_CAP", "2")
+    monkeypatch.setenv("WHIR_EXEC_LARGE_LOG", "23")
+    insts, expected = mix
+    big = instance(99, 23, num_points=1)
+    big_exp = plain(big)
+    ex = GP.Executor(workers=8)
+    try:
+        jobs = []
+        for k, (inst, exp) in enumerate(zip([big] + insts, [big_exp] + expected)):
+            t = GP.Transcript(inst["pid"], inst["sid"])
+            jobs.append((ex.submit(inst["cfg"], t, inst["dv"], inst["pts"], inst["evals"]), t, inst, exp))
+        spans = []
+        for jid, t, inst, (exp_proof, exp_ep) in jobs:
+            times, ep = ex.wait(jid)
+            spans.append((times[1], times[2]))
+            assert t.proof() == exp_proof, f"proof differs for instance seed={inst['seed']} nv={inst['nv']}"
+            assert np.array_equal(ep, exp_ep)
+    finally:
+        ex.close()
+    # a small job that started while the large one ran found nothing else running
+    (s_big, e_big), small = spans[0], spans[1:]
+    for j, (s_j, _) in enumerate(small):
+        if s_big < s_j < e_big:
+            others = sum(1 for i, (s_i, e_i) in enumerate(small) if i != j and s_i <= s_j < e_i)
+            assert others == 0, "more jobs ran beside the large one than the cap allows"
+
+
 def test_executor_mix_matches_oracle_and_verifies(mix):
     """One instance of the mix against the oracle prover and verifier."""
     insts, expected = mix
