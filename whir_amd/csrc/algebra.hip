@@ -660,9 +660,21 @@ __global__ __launch_bounds__(RT) void k_cov(uint64_t* acc, size_t n, int accumul
   }
 }
 
-// Lanes per element run: the largest K <= 16 that still gives >= 1024 blocks.
-static int cov_k(size_t n) {
-  int K = 16;
+// Elements per lane: the largest K (up to a per-degree maximum) that still gives
+// >= 1024 blocks.
+// The largest K is 16 for Fp, 8 for Fp2 and 4 for Fp3 (fewer accumulators per lane,
+// more waves: the d = 24 Fp2 covector 0.198 -> 0.177 ms, the d = 26 Fp3 ones 3.53 ->
+// 2.82 ms on MI355X); WHIR_COV_KMAX2 / WHIR_COV_KMAX3 override them.
+static int cov_k(size_t n, int D) {
+  static const int kmax2 = [] {
+    const char* e = getenv("WHIR_COV_KMAX2");
+    return e ? atoi(e) : 8;
+  }();
+  static const int kmax3 = [] {
+    const char* e = getenv("WHIR_COV_KMAX3");
+    return e ? atoi(e) : 4;
+  }();
+  int K = D == 3 ? kmax3 : (D == 2 ? kmax2 : 16);
   while (K > 1 && (n + (size_t)RT * K - 1) / ((size_t)RT * K) < 1024) K >>= 1;
   return K;
 }
@@ -677,7 +689,7 @@ static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t*
     if (!accumulate) fill_zero(acc, n * D * 8, s);
     return;
   }
-  const int K = cov_k(n);
+  const int K = cov_k(n, D);
   CovPlan L{};
   L.np = np;
   L.nv = np ? nv : 0;
