@@ -113,32 +113,6 @@ def test_gpu_proof_with_pow(ud, sec, pb, engine):
     WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, evals, points)
 
 
-def test_gpu_prover_full_size_accepts():
-    # d = 20, Goldilocks2, rate 1/2, Blake3 (the reference CLI's default shape), verified by the oracle verifier.
-    nv, D = 20, 2
-    pid, sid, vec, points, params = make_case(nv, D, W.HASH_BLAKE3, 4, 4, 1, 1, sec=100, pow_bits=20)
-    cfg = WP.WhirConfig.new(1 << nv, WP.ProtocolParameters(**params), D)
-    cfg.disable_pow()
-    gevals, gnarg, ghints, gep = gpu_proof(nv, D, pid, sid, vec, points, params, True)
-    assert np.array_equal(gevals[0], WP.mle_evaluate_base(D, vec, points[0]))
-    WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, gevals, points)
-
-
-@pytest.mark.parametrize("nv,npts", [(26, 1), (22, 0)])
-def test_gpu_prover_goldilocks3_sha3_with_pow_accepts(nv, npts):
-    """Goldilocks3, rate 1/2, folding 4, SHA3 Merkle, 100-bit security with the
-    configuration's own PoW (the query-phase grinding runs; the 192-bit field needs
-    no fold PoW): PCS mode at d = 26 (a 2^27-row codeword per coset pair) and LDT
-    mode (no evaluation points), each accepted by the oracle verifier."""
-    D = 3
-    pid, sid, vec, points, params = make_case(nv, D, W.HASH_SHA3, 4, 4, 1, npts, sec=100, pow_bits=20)
-    cfg = WP.WhirConfig.new(1 << nv, WP.ProtocolParameters(**params), D)
-    gevals, gnarg, ghints, gep = gpu_proof(nv, D, pid, sid, vec, points, params, False)
-    for k in range(npts):
-        assert np.array_equal(gevals[k], WP.mle_evaluate_base(D, vec, points[k]))
-    WP.verify(WP.TranscriptReader(pid, sid, gnarg, ghints), cfg, gevals, points)
-
-
 @pytest.mark.parametrize("kind", ["zeros", "max", "iota", "ones"])
 @pytest.mark.parametrize("nv,D,k0,k,lir", [(12, 2, 4, 4, 1), (5, 3, 1, 1, 1), (4, 2, 4, 4, 2), (9, 1, 3, 2, 1)])
 def test_gpu_proof_special_vectors(kind, nv, D, k0, k, lir):

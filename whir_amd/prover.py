@@ -101,6 +101,86 @@ class Transcript:
             pass
 
 
+class _Vtable(ctypes.Structure):
+    _F = ctypes.CFUNCTYPE
+    _u8p = ctypes.POINTER(ctypes.c_uint8)
+    _fields_ = [("ctx", vp),
+                ("prover_field", _F(ci, vp, ci, u64p, sz)),
+                ("verifier_field", _F(ci, vp, ci, u64p)),
+                ("verifier_bytes", _F(ci, vp, _u8p, sz)),
+                ("prover_hash", _F(ci, vp, _u8p)),
+                ("prover_u64", _F(ci, vp, ctypes.c_uint64)),
+                ("hint_hash", _F(ci, vp, _u8p)),
+                ("hint_field_vec", _F(ci, vp, ci, u64p, sz))]
+
+
+class CallbackTranscript:
+    """A transcript whose typed operations (ProverState, src/transcript/mod.rs:183-267)
+    run in Python, through whir_transcript_from_vtable: the GPU prover calls
+    impl.prover_field(D, words), impl.verifier_field(D) -> D words,
+    impl.verifier_bytes(n) -> n bytes, impl.prover_hash(h32), impl.prover_u64(v),
+    impl.hint_hash(h32) and impl.hint_field_vec(D, words) in transcript order.  An
+    exception in a callback fails the prover call (whir_last_error names the
+    operation); the first one is kept in `error`."""
+
+    def __init__(self, impl):
+        self.impl = impl
+        self.error = None
+
+        def guard(fn):
+            def run(*a):
+                try:
+                    fn(*a)
+                    return 0
+                except Exception as e:  # reported through the status, kept for the caller
+                    if self.error is None:
+                        self.error = e
+                    return 1
+            return run
+
+        def prover_field(_, D, p, count):
+            impl.prover_field(D, np.ctypeslib.as_array(p, shape=(D * count,)).copy() if count else
+                              np.zeros(0, np.uint64))
+
+        def verifier_field(_, D, out):
+            v = np.asarray(impl.verifier_field(D), dtype=np.uint64).reshape(-1)
+            assert v.size == D
+            np.ctypeslib.as_array(out, shape=(D,))[:] = v
+
+        def verifier_bytes(_, out, n):
+            b = bytes(impl.verifier_bytes(n))
+            assert len(b) == n
+            if n:
+                ctypes.memmove(out, b, n)
+
+        def prover_hash(_, h):
+            impl.prover_hash(ctypes.string_at(h, 32))
+
+        def prover_u64(_, v):
+            impl.prover_u64(int(v))
+
+        def hint_hash(_, h):
+            impl.hint_hash(ctypes.string_at(h, 32))
+
+        def hint_field_vec(_, D, p, count):
+            impl.hint_field_vec(D, np.ctypeslib.as_array(p, shape=(D * count,)).copy() if count else
+                                np.zeros(0, np.uint64))
+
+        fns = [prover_field, verifier_field, verifier_bytes, prover_hash, prover_u64, hint_hash, hint_field_vec]
+        self._vt = _Vtable(None, *[ftype(guard(fn)) for fn, (_, ftype) in zip(fns, _Vtable._fields_[1:])])
+        h = vp()
+        _call("whir_transcript_from_vtable", ctypes.byref(self._vt), ctypes.byref(h))
+        self.h = h.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().whir_transcript_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class Witness:
     def __init__(self, h):
         self.h = h
