@@ -299,3 +299,85 @@ def test_executor_batch_job_rejects_a_partial_batch():
                             bi["pts"], bi["evals"][:, :1])
     finally:
         ex.close()
+
+
+def _hip_used_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return total.value - free.value
+
+
+def test_executor_device_memory_stays_within_budget(mix):
+    """Mixed sizes under a budget of about two of the largest jobs: the device memory
+    in use (hipMemGetInfo, sampled while the jobs run) stays within the budget plus a
+    small allowance for pool granularity, because the buffers streams keep cached
+    count against the budget and idle workers' caches go back to the device when a
+    job needs the room.  Every proof is unchanged."""
+    import threading
+    import time
+    insts, expected = mix
+    mems = [GP.config_memory(i["cfg"]) for i in insts]
+    budget = 2 * max(mems) + max(mems) // 2
+    W.synchronize()
+    base = _hip_used_bytes()
+    peak = [base]
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            peak[0] = max(peak[0], _hip_used_bytes())
+            time.sleep(0.0005)
+
+    th = threading.Thread(target=sample)
+    th.start()
+    ex = GP.Executor(workers=6, hbm_budget=budget)
+    try:
+        jobs = []
+        for rep in range(3):
+            for inst, exp in zip(insts[::-1], expected[::-1]):
+                t = GP.Transcript(inst["pid"], inst["sid"])
+                jobs.append((ex.submit(inst["cfg"], t, inst["dv"], inst["pts"], inst["evals"]), t, exp))
+        for jid, t, (exp_proof, _) in jobs:
+            ex.wait(jid)
+            assert t.proof() == exp_proof
+    finally:
+        ex.close()
+        stop.set()
+        th.join()
+    allowance = (64 << 20) + budget // 20
+    assert peak[0] - base <= budget + allowance, \
+        f"device memory grew by {(peak[0] - base) >> 20} MiB under a {budget >> 20} MiB budget"
+
+
+def test_executor_head_of_queue_is_not_passed_forever(mix, monkeypatch):
+    """A job that fits the budget only when nothing else runs, submitted ahead of a
+    stream of small jobs: at most WHIR_EXEC_MAX_PASS of them start before it (without
+    the bound every one of them would), and every proof is unchanged."""
+    monkeypatch.setenv("WHIR_EXEC_MAX_PASS", "3")
+    insts, expected = mix
+    big, big_exp = insts[-1], expected[-1]
+    small = [(insts[k % 4], expected[k % 4]) for k in range(40)]
+    budget = GP.config_memory(big["cfg"])
+    ex = GP.Executor(workers=4, hbm_budget=budget)
+    try:
+        first = small[0]
+        t0 = GP.Transcript(first[0]["pid"], first[0]["sid"])
+        jobs = [(ex.submit(first[0]["cfg"], t0, first[0]["dv"], first[0]["pts"], first[0]["evals"]), t0, first[1],
+                 "s")]
+        tb = GP.Transcript(big["pid"], big["sid"])
+        jobs.append((ex.submit(big["cfg"], tb, big["dv"], big["pts"], big["evals"]), tb, big_exp, "big"))
+        for inst, exp in small[1:]:
+            t = GP.Transcript(inst["pid"], inst["sid"])
+            jobs.append((ex.submit(inst["cfg"], t, inst["dv"], inst["pts"], inst["evals"]), t, exp, "s"))
+        starts = []
+        for jid, t, (exp_proof, _), tag in jobs:
+            times, _ = ex.wait(jid)
+            starts.append((times[1], tag))
+            assert t.proof() == exp_proof
+    finally:
+        ex.close()
+    big_start = [s for s, tag in starts if tag == "big"][0]
+    passed = sum(1 for s, tag in starts[2:] if tag == "s" and s < big_start)
+    assert passed <= 3, f"{passed} later jobs started before the waiting head of the queue"

@@ -56,10 +56,17 @@ uint64_t* result_mailbox(hipStream_t s);
 // free list and is handed out again only to allocations on s, so stream order
 // alone keeps its last readers ahead of its next writers.  Sizes repeat from
 // one proof to the next, so after the first proof nothing is allocated.
+// Allocations come from the library's own stream-ordered pool (one per device); an
+// allocation that finds the device out of memory first returns every stream's cached
+// buffers and tries once more.
 void* pool_alloc(size_t bytes, hipStream_t s);
 void pool_free(void* p, size_t bytes, hipStream_t s);
-// Drop every cached buffer of stream s (e.g. before the stream is destroyed).
+// Drop every cached buffer of stream s (e.g. before the stream is destroyed), or of
+// every stream; the freed memory goes back to the device.
 void pool_trim(hipStream_t s);
+void pool_trim_all();
+// Bytes held in the free lists of stream s (all streams when s is null).
+size_t pool_cached_bytes(hipStream_t s);
 
 // Stream-ordered device scratch that returns to the pool on scope exit.
 struct DevBuf {

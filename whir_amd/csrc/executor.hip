@@ -14,7 +14,12 @@
 // Admission: a job's peak device memory is estimated from its configuration
 // (job_memory); a worker starts the first queued job that fits in what the
 // budget has left, so small requests pass a large one that is waiting for
-// memory.  A job larger than the whole budget runs only when nothing else does.
+// memory, but only a bounded number of times: once the job at the head of the
+// queue has been passed max_pass times, no other job starts until it has.  What
+// the budget has left counts the running jobs' estimates and the buffers the
+// streams keep cached for reuse; when a job fits only without the idle workers'
+// caches, those are handed back to the device first.  A job larger than the whole
+// budget runs only when nothing else does.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,10 +44,11 @@ namespace whir {
 namespace {
 
 // Peak device bytes of one commit + prove: the covector and folded vector, the
-// initial codeword (plus its NTT scratch) and Merkle nodes, and the largest
-// round witness alive next to it.
-// With several commitments (multi-vector jobs) every initial witness stays alive
-// until the first round has opened it.
+// initial codeword (plus its NTT scratch) and Merkle nodes, which the caller keeps
+// for the whole proof, and the largest pair of consecutive round witnesses (a
+// round's witness is committed while the previous round's, which that round opens,
+// is still alive).  With several commitments (multi-vector jobs) every initial
+// witness stays alive until the first round has opened it.
 size_t job_memory(const WhirCfg& c, size_t commitments = 1) {
   auto wit = [](const IrsCfg& irs, int Ds) {
     const size_t N = irs.codeword_length, M = irs.num_messages();
@@ -50,7 +56,11 @@ size_t job_memory(const WhirCfg& c, size_t commitments = 1) {
   };
   const size_t n = c.initial.vector_size;
   size_t mx = 0;
-  for (const RoundCfg& r : c.rounds) mx = std::max(mx, wit(r.irs, c.D));
+  for (size_t k = 0; k < c.rounds.size(); k++) {
+    size_t w = wit(c.rounds[k].irs, c.D);
+    if (k) w += wit(c.rounds[k - 1].irs, c.D);
+    mx = std::max(mx, w);
+  }
   return 2 * n * (size_t)c.D * 8 + commitments * wit(c.initial, 1) + mx;
 }
 
@@ -74,6 +84,7 @@ struct Job {
   std::string error;
   std::vector<uint64_t> eval_point;
   double t_submit = 0, t_start = 0, t_end = 0;
+  int passed = 0;  // times a later job started while this one waited at the head of the queue
 };
 
 }  // namespace
@@ -93,20 +104,59 @@ struct whir_executor {
   // requests/s, d log-uniform [14, 24], 8 workers: p99 13.6 -> 12.6-13.1 ms, p50 +2-4%;
   // vectors of 2^23 and up: the 2^16..2^22 batch keeps its throughput).
   int large_log = 23, large_cap = 6;
+  // The job at the head of the queue may be passed by later jobs this many times
+  // (WHIR_EXEC_MAX_PASS; 0 = strict order).
+  int max_pass = 16;
   uint64_t next_id = 1;
   bool stop = false;
   int device = 0;
+  std::vector<hipStream_t> streams;  // per worker, set when the worker starts
+  std::vector<char> busy;            // per worker: running a job
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   bool is_large(const Job* j) const { return j->cfg->cfg.initial.vector_size >= ((size_t)1 << large_log); }
+
+  // Cached bytes of the idle workers' streams (buffers no running job can be using).
+  size_t idle_cached() const {
+    size_t b = 0;
+    for (size_t i = 0; i < streams.size(); i++)
+      if (!busy[i] && streams[i]) b += pool_cached_bytes(streams[i]);
+    return b;
+  }
+  void trim_idle() {
+    for (size_t i = 0; i < streams.size(); i++)
+      if (!busy[i] && streams[i]) pool_trim(streams[i]);
+  }
+
+  // Whether j can start now (called with mu held): always when nothing runs, else
+  // when the running estimates, the cached buffers and j fit the budget, handing the
+  // idle workers' caches back to the device when only that makes it fit.
+  bool admit(const Job* j) {
+    const size_t cached = pool_cached_bytes(nullptr);
+    if (in_use + j->mem + cached <= budget) return true;
+    const size_t idle = idle_cached();
+    if (running == 0 || in_use + j->mem + (cached - std::min(cached, idle)) <= budget) {
+      if (idle) trim_idle();
+      return true;
+    }
+    return false;
+  }
 
   // First queued job that fits the remaining budget (or any job when idle).  With a
   // large-job cap, no job starts while a large one runs beside large_cap - 1 others.
   Job* pick() {
     if (large_cap > 0 && running_large > 0 && running >= large_cap) return nullptr;
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
+    if (queue.empty()) return nullptr;
+    Job* head = queue.front();
+    if (admit(head)) {
+      queue.pop_front();
+      return head;
+    }
+    if (head->passed >= max_pass) return nullptr;  // the head has waited long enough: drain for it
+    for (auto it = queue.begin() + 1; it != queue.end(); ++it) {
       Job* j = *it;
-      if (running == 0 || in_use + j->mem <= budget) {
+      if (admit(j)) {
+        head->passed++;
         queue.erase(it);
         return j;
       }
@@ -156,8 +206,17 @@ struct whir_executor {
     j->eval_point.swap(ep);
   }
 
-  void loop() {
+  void loop(size_t wi) {
     (void)hipSetDevice(device);
+    {
+      hipStream_t s = nullptr;
+      try {
+        s = thread_stream();
+      } catch (...) {
+      }
+      std::lock_guard<std::mutex> g(mu);
+      streams[wi] = s;
+    }
     for (;;) {
       Job* j = nullptr;
       {
@@ -166,6 +225,7 @@ struct whir_executor {
         if (!j) break;  // stop requested and nothing admissible left
         in_use += j->mem;
         running++;
+        busy[wi] = 1;
         if (is_large(j)) running_large++;
         j->t_start = now_s(t0);
       }
@@ -176,10 +236,15 @@ struct whir_executor {
         j->done = true;
         in_use -= j->mem;
         running--;
+        busy[wi] = 0;
         if (is_large(j)) running_large--;
       }
       cv_done.notify_all();
       cv_work.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> g(mu);
+      streams[wi] = nullptr;  // its cache goes with it
     }
     release_thread_resources();
   }
@@ -224,8 +289,11 @@ int whir_executor_new(int num_workers, size_t hbm_budget, whir_executor** out) {
   // starts while WHIR_EXEC_LARGE_CAP jobs run (0 turns the cap off).
   if (const char* v = getenv("WHIR_EXEC_LARGE_CAP")) e->large_cap = atoi(v);
   if (const char* v = getenv("WHIR_EXEC_LARGE_LOG")) e->large_log = atoi(v);
+  if (const char* v = getenv("WHIR_EXEC_MAX_PASS")) e->max_pass = atoi(v);
+  e->streams.assign(num_workers, nullptr);
+  e->busy.assign(num_workers, 0);
   whir_executor* p = e.get();
-  for (int i = 0; i < num_workers; i++) e->workers.emplace_back([p] { p->loop(); });
+  for (int i = 0; i < num_workers; i++) e->workers.emplace_back([p, i] { p->loop((size_t)i); });
   *out = e.release();
   EXEC_API_END
 }

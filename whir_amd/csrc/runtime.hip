@@ -30,20 +30,28 @@ thread_local ThreadStream t_stream;
 
 namespace {
 std::mutex g_pool_mu;
-std::vector<int> g_pool_ready;
+std::vector<hipMemPool_t> g_pools;  // per device: the library's own stream-ordered pool
 
-// Keep freed stream-ordered memory cached in the device pool: with the default
-// release threshold (0) every synchronization hands it back and the next proof
-// maps its half-GiB codewords again.
-void configure_pool(int dev) {
+// The library allocates from a pool of its own, so that keeping freed memory cached
+// (release threshold UINT64_MAX: with the default 0 every synchronization hands it
+// back and the next proof maps its half-GiB codewords again) leaves the device's
+// default pool, which the host application may use, as it is.
+hipMemPool_t device_pool(int dev) {
   std::lock_guard<std::mutex> g(g_pool_mu);
-  for (int d : g_pool_ready)
-    if (d == dev) return;
-  hipMemPool_t pool;
-  WHIR_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-  uint64_t thr = UINT64_MAX;
-  WHIR_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
-  g_pool_ready.push_back(dev);
+  if ((size_t)dev >= g_pools.size()) g_pools.resize(dev + 1, nullptr);
+  if (!g_pools[dev]) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    WHIR_HIP(hipMemPoolCreate(&pool, &props));
+    uint64_t thr = UINT64_MAX;
+    WHIR_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    g_pools[dev] = pool;
+  }
+  return g_pools[dev];
 }
 
 // Per-thread pinned staging area for small host->device uploads, so they can be
@@ -71,7 +79,16 @@ namespace {
 std::mutex g_cache_mu;
 // (stream, size rounded to 256 B) -> free buffers
 std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> g_cache;
+std::map<hipStream_t, size_t> g_cached_bytes;  // bytes held in each stream's free lists
+size_t g_cached_total = 0;
 size_t round_bytes(size_t b) { return (b + 255) & ~(size_t)255; }
+
+hipError_t alloc_from_pool(void** p, size_t rb, hipStream_t s) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  return hipMallocFromPoolAsync(p, rb, device_pool(dev), s);
+}
 }  // namespace
 
 void* pool_alloc(size_t bytes, hipStream_t s) {
@@ -82,35 +99,77 @@ void* pool_alloc(size_t bytes, hipStream_t s) {
     if (it != g_cache.end() && !it->second.empty()) {
       void* p = it->second.back();
       it->second.pop_back();
+      g_cached_bytes[s] -= rb;
+      g_cached_total -= rb;
       return p;
     }
   }
   void* p = nullptr;
-  WHIR_HIP(hipMallocAsync(&p, rb, s));
+  hipError_t e = alloc_from_pool(&p, rb, s);
+  if (e == hipErrorOutOfMemory) {
+    // Buffers cached by this and other streams hold device memory nobody uses: hand
+    // them back (each on its own stream, after that stream's earlier work) and try once more.
+    (void)hipGetLastError();
+    pool_trim_all();
+    e = alloc_from_pool(&p, rb, s);
+  }
+  if (e != hipSuccess)
+    throw Error(std::string("device allocation of ") + std::to_string(rb) + " bytes: " + hipGetErrorString(e));
   return p;
 }
 
 void pool_free(void* p, size_t bytes, hipStream_t s) {
+  const size_t rb = round_bytes(bytes);
   std::lock_guard<std::mutex> g(g_cache_mu);
-  g_cache[{s, round_bytes(bytes)}].push_back(p);
+  g_cache[{s, rb}].push_back(p);
+  g_cached_bytes[s] += rb;
+  g_cached_total += rb;
+}
+
+size_t pool_cached_bytes(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  if (!s) return g_cached_total;
+  auto it = g_cached_bytes.find(s);
+  return it == g_cached_bytes.end() ? 0 : it->second;
+}
+
+// Takes every cached buffer of stream s (all streams when s is null) out of the lists.
+static std::vector<std::pair<hipStream_t, void*>> take_cached(hipStream_t s) {
+  std::vector<std::pair<hipStream_t, void*>> drop;
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end();) {
+    if (!s || it->first.first == s) {
+      for (void* p : it->second) drop.emplace_back(it->first.first, p);
+      g_cached_bytes[it->first.first] -= it->first.second * it->second.size();
+      g_cached_total -= it->first.second * it->second.size();
+      it = g_cache.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return drop;
+}
+
+static void free_taken(const std::vector<std::pair<hipStream_t, void*>>& drop) {
+  std::vector<hipStream_t> streams;
+  for (const auto& d : drop) {
+    (void)hipFreeAsync(d.second, d.first);
+    if (std::find(streams.begin(), streams.end(), d.first) == streams.end()) streams.push_back(d.first);
+  }
+  for (hipStream_t st : streams) (void)hipStreamSynchronize(st);
+  if (!drop.empty()) {
+    int dev = 0;
+    // Give the freed memory back to the device too (the pool would otherwise keep it
+    // reserved for its next allocations).
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipMemPoolTrimTo(device_pool(dev), 0);
+  }
 }
 
 void pool_trim(hipStream_t s) {
-  std::vector<void*> drop;
-  {
-    std::lock_guard<std::mutex> g(g_cache_mu);
-    for (auto it = g_cache.begin(); it != g_cache.end();) {
-      if (it->first.first == s) {
-        drop.insert(drop.end(), it->second.begin(), it->second.end());
-        it = g_cache.erase(it);
-      } else {
-        ++it;
-      }
-    }
-  }
-  for (void* p : drop) (void)hipFreeAsync(p, s);
-  if (!drop.empty()) (void)hipStreamSynchronize(s);
+  if (s) free_taken(take_cached(s));
 }
+
+void pool_trim_all() { free_taken(take_cached(nullptr)); }
 
 static void release_stream_slots(hipStream_t s);
 
@@ -130,7 +189,7 @@ hipStream_t thread_stream() {
   WHIR_HIP(hipGetDevice(&dev));
   if (!t_stream.s || t_stream.device != dev) {
     if (t_stream.s) (void)hipStreamDestroy(t_stream.s);
-    configure_pool(dev);
+    (void)device_pool(dev);
     WHIR_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
     t_stream.device = dev;
   }
