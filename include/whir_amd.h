@@ -183,6 +183,17 @@ int whir_fold_and_compute_polynomial(int degree, uint64_t *a, uint64_t *b, size_
                                      uint64_t *out, size_t *new_len);
 int whir_fold_and_compute_polynomial_dev(int degree, uint64_t *a, uint64_t *b, size_t n,
                                          const uint64_t *weight, uint64_t *out, size_t *new_len);
+/* Allocation-free variant of whir_fold_and_compute_polynomial_dev.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors; it calls no hipMalloc / hipFree / allocating
+ * routine.  Its kernels keep their block partials and result in one static device
+ * array per GPU (98,352 bytes: at most 2048 partials of 2 x 3 words plus the result),
+ * held until the process ends; the partial count is capped by the kernel grid, so
+ * every length up to 2^62 fits (longer ones are rejected).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU race. */
+int whir_fold_and_compute_polynomial_dev_noalloc(int degree, uint64_t *a, uint64_t *b, size_t n,
+                                                 const uint64_t *weight, uint64_t *out, size_t *new_len);
 /* eval_eq (multilinear.rs:170-190): acc has 2^k elements, point k elements. */
 int whir_eval_eq(int degree, uint64_t *acc, const uint64_t *point, size_t k, const uint64_t *scalar);
 int whir_eval_eq_dev(int degree, uint64_t *acc, const uint64_t *point, size_t k, const uint64_t *scalar);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "whir_fold_dev": [ci, vp, sz, u64p, szp],
     "whir_fold_and_compute_polynomial": [ci, u64p, u64p, sz, u64p, u64p, szp],
     "whir_fold_and_compute_polynomial_dev": [ci, vp, vp, sz, u64p, u64p, szp],
+    "whir_fold_and_compute_polynomial_dev_noalloc": [ci, vp, vp, sz, u64p, u64p, szp],
     "whir_eval_eq": [ci, u64p, u64p, sz, u64p],
     "whir_eval_eq_dev": [ci, vp, u64p, sz, u64p],
     "whir_geometric_accumulate": [ci, u64p, sz, u64p, u64p, sz],

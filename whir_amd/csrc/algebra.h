@@ -20,6 +20,14 @@ size_t fold_dev(int D, uint64_t* v, size_t n, const uint64_t* w, hipStream_t st)
 size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* dev_out,
                          hipStream_t st);
 
+// fold_and_poly_dev with the block partials in a caller-provided device area of at
+// least fold_poly_parts_words(D, n) words (no allocation; parts may be null when that
+// is 0).  The partial count is capped by the kernel's grid (at most 2048 blocks of
+// 2 D words), whatever n is.
+size_t fold_poly_parts_words(int D, size_t n);
+size_t fold_and_poly_ws_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* dev_out,
+                            uint64_t* parts, hipStream_t st);
+
 // The same three with a first operand of Ds words per element (Ds = 1: the
 // base-field vector of the initial sumcheck round, read without lifting it).
 // Folding writes the folded a to a_out (Ds = D: may be a_in itself).

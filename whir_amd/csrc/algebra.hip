@@ -314,9 +314,18 @@ __global__ __launch_bounds__(RT) void k_fold_poly(const uint64_t* a_in, uint64_t
   block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D);
 }
 
+// Block partials of one fold + polynomial pass: none when one block does it all.
+static unsigned fold_poly_grid(size_t n) {
+  if (n <= 2) return 1;
+  const size_t q = (next_pow2(n) >> 1) >> 1;
+  return q <= SMALL_PAIRS ? 1u : grid_for(q);
+}
+
+// parts_buf: caller-provided room for the block partials (fold_poly_parts_words), or
+// null to take it from the stream's pool.
 template <int Ds, int D>
 static size_t fold_poly_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n, const uint64_t* w,
-                          uint64_t* out, hipStream_t s) {
+                          uint64_t* out, hipStream_t s, uint64_t* parts_buf = nullptr) {
   if (n <= 2) {
     // folded length <= 1: fold then the one-element polynomial
     size_t m = fold_t<Ds, D>(a_in, a_out, n, w, s);
@@ -327,15 +336,32 @@ static size_t fold_poly_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, si
   size_t h = next_pow2(n) >> 1, nhi = n - h;
   Ext<D> ww = host_ext<D>(w);
   Ext<D> om = gl::esub<D>(gl::ext_base<D>(1), ww);
-  const unsigned g = (h >> 1) <= SMALL_PAIRS ? 1u : grid_for(h >> 1);
-  DevBuf parts(g == 1 ? 0 : (size_t)g * 2 * D * 8, s);
+  const unsigned g = fold_poly_grid(n);
+  DevBuf parts(g == 1 || parts_buf ? 0 : (size_t)g * 2 * D * 8, s);
+  uint64_t* pp = parts_buf ? parts_buf : parts.as<uint64_t>();
   {
     ProfScope prof(kernel_name("k_fold_poly", Ds, D), s, 8.0 * ((Ds + D) * (double)n + 2.0 * D * h));
-    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, g == 1 ? out : parts.as<uint64_t>());
+    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, g == 1 ? out : pp);
     WHIR_HIP(hipGetLastError());
   }
-  if (g > 1) finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
+  if (g > 1) finish_reduce<D, 2>(pp, g, out, s);
   return h;
+}
+
+size_t fold_poly_parts_words(int D, size_t n) {
+  const unsigned g = fold_poly_grid(n);
+  return g == 1 ? 0 : (size_t)g * 2 * D;
+}
+
+size_t fold_and_poly_ws_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w, uint64_t* out,
+                            uint64_t* parts, hipStream_t s) {
+  WHIR_REQUIRE(fold_poly_parts_words(D, n) == 0 || parts, "fold_and_poly: no room for the block partials");
+  switch (D) {
+    case 1: return fold_poly_t<1, 1>(a, a, b, n, w, out, s, parts);
+    case 2: return fold_poly_t<2, 2>(a, a, b, n, w, out, s, parts);
+    case 3: return fold_poly_t<3, 3>(a, a, b, n, w, out, s, parts);
+  }
+  throw Error("fold: extension degree must be 1, 2 or 3");
 }
 
 size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t n,

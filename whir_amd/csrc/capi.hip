@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/whir_amd.h"
@@ -456,6 +457,58 @@ int whir_fold_and_compute_polynomial_dev(int D, uint64_t* a, uint64_t* b, size_t
   DevBuf o(2 * D * 8, s);
   *new_len = fold_and_poly_dev(D, a, b, n, w, o.as<uint64_t>(), s);
   WHIR_HIP(hipMemcpyAsync(out, o.ptr, 2 * D * 8, hipMemcpyDeviceToHost, s));
+  WHIR_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free sumcheck round
+// Device scratch of whir_fold_and_compute_polynomial_dev_noalloc: the block partials
+// of the fused pass (at most 2048 blocks x 2 values x 3 words) and the 2 D-word
+// result.  One copy per GPU, held until the process ends; every call on a GPU shares
+// it, so calls that overlap on one GPU race.
+static constexpr size_t NOALLOC_PART_WORDS = 2048 * 2 * 3;
+static constexpr size_t NOALLOC_WORDS = NOALLOC_PART_WORDS + 2 * 3;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<uint64_t*> g_scratch_ptr;  // per device: the array's device address (null until looked up)
+
+// The device address of `scratch` on the current device, looked up once per device
+// with that device current (host code must never use the array's name as an address).
+uint64_t* scratch_on_current_device() {
+  int dev = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    WHIR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    WHIR_REQUIRE(p, "noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = (uint64_t*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int whir_fold_and_compute_polynomial_dev_noalloc(int D, uint64_t* a, uint64_t* b, size_t n, const uint64_t* w,
+                                                 uint64_t* out, size_t* new_len) {
+  API_BEGIN
+  check_degree(D);
+  WHIR_REQUIRE(a && b && w && out && new_len, "fold_and_compute_polynomial_noalloc: null argument");
+  WHIR_REQUIRE(n <= ((size_t)1 << 62), "fold_and_compute_polynomial_noalloc: length above 2^62");
+  const size_t need = fold_poly_parts_words(D, n);
+  WHIR_REQUIRE(need <= NOALLOC_PART_WORDS,
+               "fold_and_compute_polynomial_noalloc: the pass needs more block partials than the scratch holds");
+  uint64_t* ws = scratch_on_current_device();
+  hipStream_t s = thread_stream();
+  uint64_t* res = ws + NOALLOC_PART_WORDS;
+  *new_len = fold_and_poly_ws_dev(D, a, b, n, w, res, need ? ws : nullptr, s);
+  WHIR_HIP(hipMemcpyAsync(out, res, 2 * D * 8, hipMemcpyDeviceToHost, s));
   WHIR_HIP(hipStreamSynchronize(s));
   API_END
 }
