@@ -41,6 +41,35 @@ static int check(const char* op, uint64_t a, uint64_t b, uint64_t got, uint64_t 
   return 0;
 }
 
+// Extension products of (x0, x1[, x2]) and (y0, y1[, y2]) from per-product reductions.
+static void r_emul2(const uint64_t* x, const uint64_t* y, uint64_t* z) {
+  z[0] = r_add(r_mul(x[0], y[0]), r_mul(7, r_mul(x[1], y[1])));
+  z[1] = r_add(r_mul(x[0], y[1]), r_mul(x[1], y[0]));
+}
+static void r_emul3(const uint64_t* x, const uint64_t* y, uint64_t* z) {
+  z[0] = r_add(r_mul(x[0], y[0]), r_mul(2, r_add(r_mul(x[1], y[2]), r_mul(x[2], y[1]))));
+  z[1] = r_add(r_add(r_mul(x[0], y[1]), r_mul(x[1], y[0])), r_mul(2, r_mul(x[2], y[2])));
+  z[2] = r_add(r_add(r_mul(x[0], y[2]), r_mul(x[1], y[1])), r_mul(x[2], y[0]));
+}
+
+static int ext(const uint64_t* x, const uint64_t* y) {
+  int bad = 0;
+  gl::Ext<2> a2{{x[0], x[1]}}, b2{{y[0], y[1]}};
+  gl::Ext<2> c2 = gl::emul<2>(a2, b2);
+  uint64_t w2[2];
+  r_emul2(x, y, w2);
+  bad |= check("emul2.c0", x[0], y[0], c2.c[0], w2[0]);
+  bad |= check("emul2.c1", x[1], y[1], c2.c[1], w2[1]);
+  gl::Ext<3> a3{{x[0], x[1], x[2]}}, b3{{y[0], y[1], y[2]}};
+  gl::Ext<3> c3 = gl::emul<3>(a3, b3);
+  uint64_t w3[3];
+  r_emul3(x, y, w3);
+  bad |= check("emul3.c0", x[0], y[0], c3.c[0], w3[0]);
+  bad |= check("emul3.c1", x[1], y[1], c3.c[1], w3[1]);
+  bad |= check("emul3.c2", x[2], y[2], c3.c[2], w3[2]);
+  return bad;
+}
+
 static int pair(uint64_t a, uint64_t b) {
   int bad = 0;
   bad |= check("add", a, b, gl::add(a, b), r_add(a, b));
@@ -76,6 +105,19 @@ int main(int argc, char** argv) {
   for (int i = 0; i < ne && !bad; i++)
     for (unsigned k = 0; k < 192 && !bad; k++)
       bad |= check("mul_pow2", edge[i], k, gl::mul_pow2(canon(edge[i]), k), r_pow2(edge[i], k));
+  // extension products over all triples of edge values (canonical), and random ones
+  for (int i = 0; i < ne && !bad; i++)
+    for (int j = 0; j < ne && !bad; j++)
+      for (int k = 0; k < ne && !bad; k += 5) {
+        uint64_t x[3] = {canon(edge[i]), canon(edge[j]), canon(edge[k])};
+        uint64_t y[3] = {canon(edge[j]), canon(edge[k]), canon(edge[i])};
+        bad |= ext(x, y);
+      }
+  for (long t = 0; t < n_random / 8 && !bad; t++) {
+    uint64_t x[3] = {canon(rnd()), canon(rnd()), canon(rnd())}, y[3] = {canon(rnd()), canon(rnd()), canon(rnd())};
+    if (t & 1) x[1] = P - 1 - (rnd() & 0xFF), y[1] = P - 1 - (rnd() & 0xFF);
+    bad |= ext(x, y);
+  }
   for (long t = 0; t < n_random && !bad; t++) {
     uint64_t a = canon(rnd()), b = canon(rnd());
     // mix in values near p and near the 32-bit boundaries

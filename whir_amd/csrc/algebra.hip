@@ -405,10 +405,20 @@ size_t fold_and_poly_dev(int D, uint64_t* a, uint64_t* b, size_t n, const uint64
 template <int Ds, int D>
 __device__ __forceinline__ Ext<D> fold_twice(const uint64_t* v, size_t t, size_t L, const Ext<D>& r1,
                                              const Ext<D>& r2) {
-  const Ext<D> x0 = ldx<Ds, D>(v, t), x1 = ldx<Ds, D>(v, L + t), x2 = ldx<Ds, D>(v, 2 * L + t),
-               x3 = ldx<Ds, D>(v, 3 * L + t);
-  const Ext<D> p = gl::eadd<D>(x0, gl::emul<D>(gl::esub<D>(x2, x0), r1));
-  const Ext<D> q = gl::eadd<D>(x1, gl::emul<D>(gl::esub<D>(x3, x1), r1));
+  Ext<D> p, q;
+  if constexpr (Ds == 1 && D > 1) {
+    // base-field source: the first fold scales r1 by a base-field difference
+    const uint64_t x0 = v[t], x1 = v[L + t], x2 = v[2 * L + t], x3 = v[3 * L + t];
+    p = gl::escale<D>(r1, gl::sub(x2, x0));
+    p.c[0] = gl::add(p.c[0], x0);
+    q = gl::escale<D>(r1, gl::sub(x3, x1));
+    q.c[0] = gl::add(q.c[0], x1);
+  } else {
+    const Ext<D> x0 = ldx<Ds, D>(v, t), x1 = ldx<Ds, D>(v, L + t), x2 = ldx<Ds, D>(v, 2 * L + t),
+                 x3 = ldx<Ds, D>(v, 3 * L + t);
+    p = gl::eadd<D>(x0, gl::emul<D>(gl::esub<D>(x2, x0), r1));
+    q = gl::eadd<D>(x1, gl::emul<D>(gl::esub<D>(x3, x1), r1));
+  }
   return gl::eadd<D>(p, gl::emul<D>(gl::esub<D>(q, p), r2));
 }
 

@@ -48,7 +48,9 @@ GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
   const uint32_t hl = (uint32_t)hi;
   uint64_t t0 = __builtin_subcl(lo, hh, 0, &c);                // lo - hh, c = borrow
   t0 -= (uint64_t)(0u - (uint32_t)c);                          // borrow: subtract 2^32 - 1 (add p)
-  const uint64_t t1 = (uint64_t)hl * 0xFFFFFFFFu;              // hl * (2^32 - 1)
+  // hl * (2^32 - 1) as one v_mad_u64_u32: the shift-and-subtract form is two
+  // full-rate ops plus a carry hazard and measured no faster on MI355X
+  const uint64_t t1 = (uint64_t)hl * 0xFFFFFFFFu;
   uint64_t t2 = __builtin_addcl(t0, t1, 0, &c);
   t2 += (uint64_t)(0u - (uint32_t)c);                          // carry: 2^64 = 2^32 - 1
   uint64_t t3 = __builtin_addcl(t2, EPS, 0, &c);               // t2 + 2^32 - 1 wraps iff t2 >= p
@@ -210,28 +212,96 @@ GL_FN Ext<1> emul<1>(const Ext<1>& a, const Ext<1>& b) {
   r.c[0] = mul(a.c[0], b.c[0]);
   return r;
 }
-// Fp2: (a0 + a1 u)(b0 + b1 u), u^2 = 7 (Karatsuba).
+// Sums of 128-bit products, reduced once: value = lo + hi 2^64 + top 2^128, where top
+// counts the carries out of hi (a handful of products, so top stays small).  With
+// 2^64 = 2^32 - 1 and 2^128 = -2^32 (mod p), the value is reduce128(lo, hi) - top 2^32,
+// and top 2^32 < p.  An extension product then costs one reduction per output
+// coefficient instead of one per partial product.
+struct Wide {
+  uint64_t lo, hi;
+  uint32_t top;
+};
+
+GL_FN Wide wide_prod(uint64_t a, uint64_t b) {
+  Wide w;
+  mul128(a, b, w.lo, w.hi);
+  w.top = 0;
+  return w;
+}
+GL_FN void wide_add(Wide& w, const Wide& x) {
+#if defined(GL_CARRY)
+  unsigned long c0, c1;
+  w.lo = __builtin_addcl(w.lo, x.lo, 0, &c0);
+  w.hi = __builtin_addcl(w.hi, x.hi, c0, &c1);
+  w.top += x.top + (uint32_t)c1;
+#else
+  const unsigned __int128 s = (((unsigned __int128)w.hi << 64) | w.lo) + (((unsigned __int128)x.hi << 64) | x.lo);
+  const bool c = (uint64_t)(s >> 64) < w.hi || ((uint64_t)(s >> 64) == w.hi && (uint64_t)s < w.lo);
+  w.lo = (uint64_t)s;
+  w.hi = (uint64_t)(s >> 64);
+  w.top += x.top + (c ? 1u : 0u);
+#endif
+}
+// w -= x, for w >= x as integers.
+GL_FN void wide_sub(Wide& w, const Wide& x) {
+#if defined(GL_CARRY)
+  unsigned long b0, b1;
+  w.lo = __builtin_subcl(w.lo, x.lo, 0, &b0);
+  w.hi = __builtin_subcl(w.hi, x.hi, b0, &b1);
+  w.top -= x.top + (uint32_t)b1;
+#else
+  const unsigned __int128 a = ((unsigned __int128)w.hi << 64) | w.lo, b = ((unsigned __int128)x.hi << 64) | x.lo;
+  w.top -= x.top + (a < b ? 1u : 0u);
+  const unsigned __int128 d = a - b;
+  w.lo = (uint64_t)d;
+  w.hi = (uint64_t)(d >> 64);
+#endif
+}
+GL_FN void wide_mac(Wide& w, uint64_t a, uint64_t b) { wide_add(w, wide_prod(a, b)); }
+// w <<= k for 1 <= k <= 3.
+GL_FN void wide_shl(Wide& w, unsigned k) {
+  w.top = (w.top << k) | (uint32_t)(w.hi >> (64 - k));
+  w.hi = (w.hi << k) | (w.lo >> (64 - k));
+  w.lo <<= k;
+}
+GL_FN uint64_t wide_reduce(const Wide& w) { return sub(reduce128(w.lo, w.hi), (uint64_t)w.top << 32); }
+
+// Fp2: (a0 + a1 u)(b0 + b1 u), u^2 = 7: c0 = a0 b0 + 8 a1 b1 - a1 b1, c1 = a0 b1 + a1 b0,
+// each a sum of 128-bit products reduced once.
 template <>
 GL_FN Ext<2> emul<2>(const Ext<2>& a, const Ext<2>& b) {
-  uint64_t v0 = mul(a.c[0], b.c[0]);
-  uint64_t v1 = mul(a.c[1], b.c[1]);
-  uint64_t m = mul(add(a.c[0], a.c[1]), add(b.c[0], b.c[1]));
+  Wide c0 = wide_prod(a.c[0], b.c[0]);
+  const Wide t = wide_prod(a.c[1], b.c[1]);
+  Wide t8 = t;
+  wide_shl(t8, 3);
+  wide_add(c0, t8);
+  wide_sub(c0, t);
+  Wide c1 = wide_prod(a.c[0], b.c[1]);
+  wide_mac(c1, a.c[1], b.c[0]);
   Ext<2> r;
-  r.c[0] = add(v0, mul7(v1));
-  r.c[1] = sub(sub(m, v0), v1);
+  r.c[0] = wide_reduce(c0);
+  r.c[1] = wide_reduce(c1);
   return r;
 }
-// Fp3: u^3 = 2.
+// Fp3: u^3 = 2: c0 = a0 b0 + 2 (a1 b2 + a2 b1), c1 = a0 b1 + a1 b0 + 2 a2 b2,
+// c2 = a0 b2 + a1 b1 + a2 b0, each reduced once.
 template <>
 GL_FN Ext<3> emul<3>(const Ext<3>& a, const Ext<3>& b) {
-  uint64_t a0b0 = mul(a.c[0], b.c[0]), a1b1 = mul(a.c[1], b.c[1]), a2b2 = mul(a.c[2], b.c[2]);
-  uint64_t a0b1 = mul(a.c[0], b.c[1]), a1b0 = mul(a.c[1], b.c[0]);
-  uint64_t a0b2 = mul(a.c[0], b.c[2]), a2b0 = mul(a.c[2], b.c[0]);
-  uint64_t a1b2 = mul(a.c[1], b.c[2]), a2b1 = mul(a.c[2], b.c[1]);
+  Wide c0 = wide_prod(a.c[1], b.c[2]);
+  wide_mac(c0, a.c[2], b.c[1]);
+  wide_shl(c0, 1);
+  wide_mac(c0, a.c[0], b.c[0]);
+  Wide c1 = wide_prod(a.c[2], b.c[2]);
+  wide_shl(c1, 1);
+  wide_mac(c1, a.c[0], b.c[1]);
+  wide_mac(c1, a.c[1], b.c[0]);
+  Wide c2 = wide_prod(a.c[0], b.c[2]);
+  wide_mac(c2, a.c[1], b.c[1]);
+  wide_mac(c2, a.c[2], b.c[0]);
   Ext<3> r;
-  r.c[0] = add(a0b0, dbl(add(a1b2, a2b1)));
-  r.c[1] = add(add(a0b1, a1b0), dbl(a2b2));
-  r.c[2] = add(add(a0b2, a2b0), a1b1);
+  r.c[0] = wide_reduce(c0);
+  r.c[1] = wide_reduce(c1);
+  r.c[2] = wide_reduce(c2);
   return r;
 }
 
