@@ -293,6 +293,18 @@ void whir_transcript_free(whir_transcript *t);
 /* whir::Config::commit of a device-resident base-field vector of `size` words.
  * The witness (codeword and Merkle tree) stays on the device. */
 int whir_commit_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *vector, whir_witness **out);
+/* whir::Config::commit of `count` independent instances (each its own configuration
+ * with batch_size 1, transcript and device vector), with results identical to `count`
+ * calls of whir_commit_dev.  No counterpart in the reference (which commits one
+ * instance per call); for callers that commit many instances.  The instances are
+ * grouped by shape and each group's Reed-Solomon encodes, leaf hashes and Merkle levels
+ * run as shared launches over device tables of the instances' buffers; the launch
+ * sequence of a batch layout (the list of shapes) is recorded as a HIP graph the first
+ * time and replayed afterwards.  The calling thread keeps the last few layouts' plans
+ * (tables, NTT scratch of the largest group) until it ends.  out receives count
+ * witnesses (free each with whir_witness_free). */
+int whir_commit_many_dev(const whir_config *const *cfgs, whir_transcript *const *transcripts,
+                         const uint64_t *const *vectors, size_t count, whir_witness **out);
 /* whir::Config::prove: `points` holds num_points points of num_variables
  * elements (degree words each), `evaluations` the claimed MLE values.
  * evaluation_point (optional) receives the FinalClaim point (num_variables elements). */
@@ -360,6 +372,12 @@ int whir_executor_submit(whir_executor *e, int kind, const whir_config *cfg, whi
  * device vectors must stay valid until the job has been waited for. */
 int whir_executor_submit_batch(whir_executor *e, const whir_config *cfg, whir_transcript *t,
                                const uint64_t *const *vectors, size_t num_vectors, const uint64_t *points,
+                               size_t num_points, const uint64_t *evaluations, uint64_t *job_id);
+/* kind 2 (prove only): whir_prove_dev for a vector already committed to in `t`
+ * (witness from whir_commit_dev or whir_commit_many_dev; it must stay valid until the
+ * job has been waited for). */
+int whir_executor_submit_prove(whir_executor *e, const whir_config *cfg, whir_transcript *t,
+                               const uint64_t *vector, const whir_witness *witness, const uint64_t *points,
                                size_t num_points, const uint64_t *evaluations, uint64_t *job_id);
 /* Blocks until the job has finished and returns its status.  times (optional)
  * receives {submitted, started, finished} on the executor's clock (seconds);

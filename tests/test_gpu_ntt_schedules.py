@@ -49,7 +49,8 @@ def test_every_candidate_schedule_matches_oracle(D, M, L, N, clean_schedules):
             for tw in tiles():
                 scheds.append((0, v, tw, 0, 4096))
     else:
-        splits = [(logC + 1) // 2] + ([logC // 2] if logC % 2 else [])
+        # the tuner's splits: the balanced one(s) and one level either side
+        splits = sorted({(logC + 1) // 2, logC // 2, (logC + 1) // 2 + 1, logC // 2 - 1})
         for c1 in splits:
             for v in range(4):
                 for tw in tiles():

@@ -129,6 +129,8 @@ SIGNATURES = {
     "whir_executor_submit_batch": [vp, vp, vp, ctypes.POINTER(vp), sz, u64p, sz, u64p,
                                    ctypes.POINTER(ctypes.c_uint64)],
     "whir_executor_wait": [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), u64p],
+    "whir_executor_submit_prove": [vp, vp, vp, vp, vp, u64p, sz, u64p, ctypes.POINTER(ctypes.c_uint64)],
+    "whir_commit_many_dev": [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), sz, ctypes.POINTER(vp)],
     "whir_executor_clock": [vp, ctypes.POINTER(ctypes.c_double)],
     "whir_mle_evaluate_dev": [ci, vp, sz, u64p, u64p],
 }

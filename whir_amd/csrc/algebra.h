@@ -71,6 +71,15 @@ void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_
 void univariate_eval_many_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, size_t m,
                               uint64_t* dev_out, hipStream_t st, bool padded = false);
 
+// mixed_univariate_evaluate of many base-field vectors, each at its own degree-D
+// point, in shared launches: out[e*D .. (e+1)*D) = sum_i jobs[e].coeffs[i] x_e^i.
+struct UniJob {
+  const uint64_t* coeffs;
+  size_t n;
+  uint64_t x[3];
+};
+void univariate_eval_batch_dev(int D, const UniJob* jobs, size_t count, uint64_t* dev_out, hipStream_t st);
+
 // mixed_univariate_evaluate (src/algebra/mod.rs:105-129): sum_i c_i x^i with c in the
 // base field (Ds = 1) or in the extension (Ds = D); result to dev_out[0..D).
 void univariate_eval_dev(int Ds, int D, const uint64_t* coeffs, size_t n, const uint64_t* x, uint64_t* dev_out,

@@ -7,6 +7,8 @@
 #include "common.h"
 #include "gl.h"
 
+#include <string.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -941,6 +943,150 @@ void univariate_eval_many_dev(int Ds, int D, const uint64_t* c, size_t n, const 
     }
   }
   throw Error("univariate_evaluate: unsupported degrees");
+}
+
+// ------------------------------------------------------------------ batched univariate evaluation
+// Evaluations f_e(x_e) = sum_i c_e[i] x_e^i of many base-field vectors (e < count),
+// each at its own point, in three launches for the whole batch: tables (x_e^q for
+// q < S and x_e^(b S) per segment, from host-computed squarings), segment sums (one
+// block per segment of S = RT * UNI_K coefficients, found by a binary search over the
+// jobs' first segments), and one reducing block per job.
+struct UniJobDev {
+  const uint64_t* c;
+  uint64_t n, nb, blk0, P, B;
+};
+
+template <int D>
+__global__ void k_uni_many_tables(uint64_t* buf, const UniJobDev* jobs, const uint64_t* pw, size_t count,
+                                  size_t entries_per_job_p) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t S = (size_t)RT * UNI_K;
+  // entries: count * S powers x^q, then every job's segment offsets x^(b S)
+  size_t j, i;
+  uint64_t ex;
+  uint64_t* dst;
+  if (e < count * S) {
+    j = e / S;
+    i = e % S;
+    ex = i;
+    dst = buf + jobs[j].P + i * D;
+  } else {
+    size_t r = e - count * S;
+    // job of segment entry r: jobs are laid out with their segment counts in order
+    size_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+      const size_t mid = (lo + hi) / 2;
+      if (jobs[mid].blk0 <= r) lo = mid; else hi = mid;
+    }
+    j = lo;
+    if (r >= jobs[j].blk0 + jobs[j].nb) return;
+    i = r - jobs[j].blk0;
+    ex = (uint64_t)i * S;
+    dst = buf + jobs[j].B + i * D;
+  }
+  (void)entries_per_job_p;
+  Ext<D> acc = gl::ext_base<D>(1);
+  for (int b = 0; ex; b++, ex >>= 1)
+    if (ex & 1) acc = gl::emul<D>(acc, ld<D>(pw, (j * UNI_PW + b)));
+  st<D>(dst, 0, acc);
+}
+
+template <int D>
+__global__ __launch_bounds__(RT) void k_uni_many(const UniJobDev* jobs, size_t count, const uint64_t* buf,
+                                                 uint64_t* parts) {
+  const size_t b = blockIdx.x;
+  size_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const size_t mid = (lo + hi) / 2;
+    if (jobs[mid].blk0 <= b) lo = mid; else hi = mid;
+  }
+  const UniJobDev J = jobs[lo];
+  const size_t seg = b - J.blk0;
+  const size_t S = (size_t)RT * UNI_K;
+  const size_t i0 = seg * S + threadIdx.x;
+  Ext<D> acc[1] = {gl::ext_zero<D>()};
+#pragma unroll
+  for (int k = 0; k < UNI_K; k++) {
+    const size_t i = i0 + (size_t)k * RT;
+    if (i >= J.n) break;
+    const size_t q = (size_t)k * RT + threadIdx.x;
+    acc[0] = gl::eadd<D>(acc[0], gl::escale<D>(ld<D>(buf + J.P, q), J.c[i]));
+  }
+  acc[0] = gl::emul<D>(acc[0], ld<D>(buf + J.B, seg));
+  block_reduce_store<D, 1>(acc, parts + b * D);
+}
+
+template <int D>
+__global__ __launch_bounds__(RT) void k_uni_many_reduce(const UniJobDev* jobs, const uint64_t* parts,
+                                                        uint64_t* out) {
+  const UniJobDev J = jobs[blockIdx.x];
+  Ext<D> v[1] = {gl::ext_zero<D>()};
+  for (size_t p = threadIdx.x; p < J.nb; p += blockDim.x) v[0] = gl::eadd<D>(v[0], ld<D>(parts, J.blk0 + p));
+  block_reduce_store<D, 1>(v, out + blockIdx.x * D);
+}
+
+template <int D>
+static void uni_batch_t(const UniJob* jobs, size_t count, uint64_t* out, hipStream_t s) {
+  const size_t S = (size_t)RT * UNI_K;
+  std::vector<UniJobDev> dj(count);
+  size_t blk = 0, words = 0;
+  for (size_t j = 0; j < count; j++) {
+    WHIR_REQUIRE(jobs[j].n > 0, "univariate batch: empty vector");
+    dj[j].c = jobs[j].coeffs;
+    dj[j].n = jobs[j].n;
+    dj[j].nb = (jobs[j].n + S - 1) / S;
+    WHIR_REQUIRE((uint64_t)(dj[j].nb - 1) * S < ((uint64_t)1 << UNI_PW), "univariate batch: vector too long");
+    dj[j].blk0 = blk;
+    blk += dj[j].nb;
+  }
+  for (size_t j = 0; j < count; j++) {
+    dj[j].P = words;
+    words += S * D;
+  }
+  for (size_t j = 0; j < count; j++) {
+    dj[j].B = words;
+    words += dj[j].nb * D;
+  }
+  // squarings x^(2^b), b < UNI_PW, per job
+  std::vector<uint64_t> pw(count * UNI_PW * D);
+  for (size_t j = 0; j < count; j++) {
+    Ext<D> p;
+    for (int d = 0; d < D; d++) p.c[d] = jobs[j].x[d];
+    for (int b = 0; b < UNI_PW; b++) {
+      for (int d = 0; d < D; d++) pw[(j * UNI_PW + b) * D + d] = p.c[d];
+      p = gl::emul<D>(p, p);
+    }
+  }
+  const size_t job_words = (count * sizeof(UniJobDev) + 7) / 8;
+  DevBuf in(job_words * 8 + pw.size() * 8, s), buf(words * 8, s), parts(blk * D * 8, s);
+  {
+    std::vector<uint64_t> host(job_words + pw.size());
+    memcpy(host.data(), dj.data(), count * sizeof(UniJobDev));
+    std::copy(pw.begin(), pw.end(), host.begin() + job_words);
+    upload_async(in.ptr, host.data(), host.size() * 8, s);
+  }
+  const UniJobDev* djobs = (const UniJobDev*)in.ptr;
+  const uint64_t* dpw = in.as<uint64_t>() + job_words;
+  const size_t entries = count * S + blk;
+  k_uni_many_tables<D><<<blocks_for(entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), djobs, dpw, count, S);
+  WHIR_HIP(hipGetLastError());
+  {
+    ProfScope prof(kernel_name("k_uni_many", 1, D), s, 8.0 * (double)blk * S);
+    k_uni_many<D><<<(unsigned)blk, RT, 0, s>>>(djobs, count, buf.as<uint64_t>(), parts.as<uint64_t>());
+    WHIR_HIP(hipGetLastError());
+  }
+  k_uni_many_reduce<D><<<(unsigned)count, RT, 0, s>>>(djobs, parts.as<uint64_t>(), out);
+  WHIR_HIP(hipGetLastError());
+}
+
+void univariate_eval_batch_dev(int D, const UniJob* jobs, size_t count, uint64_t* out, hipStream_t s) {
+  if (!count) return;
+  switch (D) {
+    case 1: uni_batch_t<1>(jobs, count, out, s); return;
+    case 2: uni_batch_t<2>(jobs, count, out, s); return;
+    case 3: uni_batch_t<3>(jobs, count, out, s); return;
+  }
+  throw Error("univariate batch: extension degree must be 1, 2 or 3");
 }
 
 void univariate_eval_dev(int Ds, int D, const uint64_t* c, size_t n, const uint64_t* x, uint64_t* out,

@@ -40,6 +40,10 @@ hipStream_t thread_stream();
 // already torn down their per-thread state).  The thread may use HIP again after.
 void release_thread_resources();
 
+// The batched-commit plans of the calling thread (prover.hip): released by
+// release_thread_resources before the thread's stream.
+void release_commit_plans();
+
 // Enqueue a host->device copy on stream s; `src` may be reused as soon as this
 // returns (small copies go through a pinned per-thread staging area).
 void upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
@@ -100,6 +104,12 @@ inline unsigned blocks_for(size_t n, unsigned threads) {
 
 // Zero a device range with a kernel (no memset nodes: the launch may be captured).
 void fill_zero(void* dst, size_t bytes, hipStream_t s);
+
+// Set while the calling thread captures its stream into a HIP graph: code that would
+// allocate, synchronize, tune or time with events must not run then (the profiler's
+// scopes record nothing).
+bool capture_active();
+void set_capture_active(bool on);
 
 // Kernel timing with HIP events on the launching stream, off unless enabled
 // (whir_profile_enable).  A scope brackets one launch of a named kernel family

@@ -70,11 +70,13 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 struct Job {
   uint64_t id = 0;
-  int kind = 0;  // 0: commit + prove, 1: commit only, 2: batch_size-vector commitments + one proof
+  int kind = 0;  // 0: commit + prove, 1: commit only, 2: batch_size-vector commitments + one proof,
+                 // 3: prove only (the witness is given)
   const whir_config* cfg = nullptr;
   whir_transcript* t = nullptr;
   const uint64_t* vector = nullptr;
   std::vector<const uint64_t*> vectors;  // kind 2
+  const whir_witness* witness = nullptr;  // kind 3
   std::vector<uint64_t> points, evals;
   size_t np = 0;
   size_t mem = 0;
@@ -188,6 +190,10 @@ struct whir_executor {
       }
       if (st) err = whir_last_error();
       for (whir_witness* w : ws) whir_witness_free(w);
+    } else if (j->kind == 3) {
+      ep.assign(rounds * (size_t)c.D, 0);
+      st = whir_prove_dev(j->cfg, j->t, j->vector, j->witness, j->points.data(), j->np, j->evals.data(), ep.data());
+      if (st) err = whir_last_error();
     } else {
       whir_witness* w = nullptr;
       st = whir_commit_dev(j->cfg, j->t, j->vector, &w);
@@ -334,6 +340,38 @@ int whir_executor_submit(whir_executor* e, int kind, const whir_config* cfg, whi
   {
     std::lock_guard<std::mutex> g(e->mu);
     WHIR_REQUIRE(!e->stop, "executor_submit: executor is shutting down");
+    j->id = e->next_id++;
+    j->t_submit = now_s(e->t0);
+    *job_id = j->id;
+    e->queue.push_back(j.get());
+    e->jobs.emplace(j->id, std::move(j));
+  }
+  e->cv_work.notify_one();
+  EXEC_API_END
+}
+
+int whir_executor_submit_prove(whir_executor* e, const whir_config* cfg, whir_transcript* t, const uint64_t* vector,
+                               const whir_witness* witness, const uint64_t* points, size_t num_points,
+                               const uint64_t* evaluations, uint64_t* job_id) {
+  EXEC_API_BEGIN
+  WHIR_REQUIRE(e && cfg && t && vector && witness && job_id, "executor_submit_prove: null argument");
+  if (num_points) WHIR_REQUIRE(points && evaluations, "executor_submit_prove: null points or evaluations");
+  auto j = std::make_unique<Job>();
+  j->kind = 3;
+  j->cfg = cfg;
+  j->t = t;
+  j->vector = vector;
+  j->witness = witness;
+  j->np = num_points;
+  size_t nv = 0;
+  while (((size_t)1 << nv) < cfg->cfg.initial.vector_size) nv++;
+  const size_t D = (size_t)cfg->cfg.D;
+  j->points.assign(points, points + num_points * nv * D);
+  j->evals.assign(evaluations, evaluations + num_points * D);
+  j->mem = job_memory(cfg->cfg);
+  {
+    std::lock_guard<std::mutex> g(e->mu);
+    WHIR_REQUIRE(!e->stop, "executor_submit_prove: executor is shutting down");
     j->id = e->next_id++;
     j->t_submit = now_s(e->t0);
     *job_id = j->id;

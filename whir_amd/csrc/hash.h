@@ -25,6 +25,13 @@ void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t 
 void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
                      uint8_t* root_out = nullptr);
 
+// commit_cols_dev of `count` codewords of the same shape in shared launches: codeword
+// i at cw_tab[i], its nodes at nodes_tab[i] (device arrays of pointers), its root to
+// roots + 32 i (device memory).  N >= 8, a power of two.  Launches only (no
+// allocation, no synchronization), so a stream capture may record it.
+void commit_cols_many_dev(int engine, int D, const uint64_t* const* cw_tab, uint8_t* const* nodes_tab, size_t count,
+                          size_t N, size_t M, uint8_t* roots, hipStream_t st);
+
 // Leaf hashes of a row-major matrix m[r*M + j].
 void hash_rows_dev(int engine, int D, const uint64_t* m, size_t rows, size_t M, uint8_t* leaves,
                    hipStream_t st);

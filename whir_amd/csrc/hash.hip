@@ -40,6 +40,17 @@ __device__ __forceinline__ void store_digest(uint8_t* out, const uint32_t h[8]) 
   o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
+// Batched commits: one launch covers `count` codewords of the same shape, `per`
+// blocks each; block b works on instance b / per (codeword cw[b / per], Merkle nodes
+// nodes[b / per], root to roots + 32 (b / per)) as block b % per of a single launch.
+// A zero table (cw == nullptr) is the single-codeword launch.
+struct BatchTab {
+  const uint64_t* const* cw = nullptr;
+  uint8_t* const* nodes = nullptr;
+  uint32_t per = 1;
+  uint8_t* roots = nullptr;
+};
+
 // Calls f(std::integral_constant<int, E>{}) for the engine E given at run time, so
 // each kernel is instantiated per engine (see hash_msg_t).
 template <class F>
@@ -82,11 +93,17 @@ static size_t LEAF_TILE_MAX = [] {
 
 template <int D, int ENG>
 __global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(const uint64_t* __restrict__ cw, size_t N, size_t M,
-                                                              uint8_t* __restrict__ leaves, int RB) {
+                                                              uint8_t* __restrict__ leaves, int RB, BatchTab tab) {
   extern __shared__ uint64_t srow[];
+  size_t blk = blockIdx.x;
+  if (tab.cw) {
+    cw = tab.cw[blk / tab.per];
+    leaves = tab.nodes[blk / tab.per];
+    blk %= tab.per;
+  }
   const size_t W = M * D;          // words per row
   const size_t stride = W + 1;     // padded LDS row stride (odd: spreads banks)
-  const size_t r0 = (size_t)blockIdx.x * RB;
+  const size_t r0 = blk * RB;
   const size_t nrows = (N - r0) < (size_t)RB ? (N - r0) : (size_t)RB;
   for (size_t e = threadIdx.x; e < nrows * M; e += blockDim.x) {
     size_t r = e % nrows, j = e / nrows;
@@ -106,7 +123,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_leaves_cols(const uint64_t* __
 // Rows wider than the LDS tile: one lane per row reading straight from HBM.
 template <int D, int ENG>
 __global__ void k_leaves_cols_wide(const uint64_t* __restrict__ cw, size_t N, size_t M,
-                                   uint8_t* __restrict__ leaves) {
+                                   uint8_t* __restrict__ leaves, BatchTab tab) {
   struct ColReader {
     const uint64_t* cw;
     size_t N, r, size;
@@ -117,7 +134,13 @@ __global__ void k_leaves_cols_wide(const uint64_t* __restrict__ cw, size_t N, si
     }
     __device__ uint32_t w32(size_t off) const { return (uint32_t)(w64(off & ~(size_t)7) >> (8 * (off & 7))); }
   };
-  size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t blk = blockIdx.x;
+  if (tab.cw) {
+    cw = tab.cw[blk / tab.per];
+    leaves = tab.nodes[blk / tab.per];
+    blk %= tab.per;
+  }
+  size_t r = blk * (size_t)blockDim.x + threadIdx.x;
   if (r >= N) return;
   ColReader rd{cw, N, r, 8 * M * D};
   uint32_t h[8];
@@ -126,21 +149,24 @@ __global__ void k_leaves_cols_wide(const uint64_t* __restrict__ cw, size_t N, si
 }
 
 template <int D>
-static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st) {
+static void leaves_cols(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* leaves, hipStream_t st,
+                        BatchTab tab = {}, size_t count = 1) {
   size_t W = M * D;
   int RB = (int)std::min<size_t>(LEAF_THREADS, 8192 / (W + 1));
   // The LDS-staged kernel coalesces the row reads, but a tile above ~20 KiB per two
   // waves caps residency; wider rows read their columns straight from HBM instead.
   if ((size_t)RB * (W + 1) * 8 > LEAF_TILE_MAX) RB = 0;
   ProfScope prof(kernel_name(RB >= 16 ? "k_leaves_cols" : "k_leaves_cols_wide", D), st,
-                 8.0 * (double)W * N + 32.0 * N);
+                 (8.0 * (double)W * N + 32.0 * N) * count);
   engine_switch(engine, [&](auto E) {
     constexpr int ENG = decltype(E)::value;
     if (RB >= 16) {
       size_t smem = (size_t)RB * (W + 1) * 8;
-      k_leaves_cols<D, ENG><<<blocks_for(N, RB), LEAF_THREADS, smem, st>>>(cw, N, M, leaves, RB);
+      tab.per = blocks_for(N, RB);
+      k_leaves_cols<D, ENG><<<tab.per * (unsigned)count, LEAF_THREADS, smem, st>>>(cw, N, M, leaves, RB, tab);
     } else {
-      k_leaves_cols_wide<D, ENG><<<blocks_for(N, 128), 128, 0, st>>>(cw, N, M, leaves);
+      tab.per = blocks_for(N, 128);
+      k_leaves_cols_wide<D, ENG><<<tab.per * (unsigned)count, 128, 0, st>>>(cw, N, M, leaves, tab);
     }
   });
   WHIR_HIP(hipGetLastError());
@@ -261,11 +287,17 @@ static int merkle_gmax_rt(int) { return 3; }
 
 template <int ENG>
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restrict__ nodes, size_t W, size_t w,
-                                                               int L, int G1, uint32_t* root_out) {
+                                                               int L, int G1, uint32_t* root_out, BatchTab tab) {
   constexpr int NH = 1 << merkle_gmax<ENG>();
   __shared__ uint32_t buf[2][MERKLE_THREADS][8];
+  size_t blk = blockIdx.x;
+  if (tab.nodes) {
+    nodes = tab.nodes[blk / tab.per];
+    if (root_out) root_out = (uint32_t*)(tab.roots + 32 * (blk / tab.per));
+    blk %= tab.per;
+  }
   size_t width = w;                                 // width of the current level
-  size_t first = (size_t)blockIdx.x << L;           // block's first node within the current level
+  size_t first = blk << L;                          // block's first node within the current level
   size_t cnt = (size_t)1 << L;                      // block's nodes in the current level
   int left = L, cur = 0;
   bool from_global = true;
@@ -315,9 +347,12 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restri
 }
 
 // Merkle levels above a level of width w (a power of two) already in place.  The
-// root also goes to root_out (32 bytes) when given.
-static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st, uint8_t* root_out) {
+// root also goes to root_out (32 bytes) when given.  With a batch table, the same
+// levels of `count` trees (root i to tab.roots + 32 i).
+static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st, uint8_t* root_out,
+                        BatchTab tab = {}, size_t count = 1) {
   if (w <= 1) {
+    WHIR_REQUIRE(!tab.nodes, "merkle: a batch of one-leaf trees is not supported");
     if (root_out) WHIR_HIP(hipMemcpyAsync(root_out, nodes + 32 * (2 * W - 2), 32, hipMemcpyDefault, st));
     return;
   }
@@ -331,10 +366,11 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     const size_t blocks = w >> L;
     const unsigned threads =
         (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
-    ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)));
+    ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)) * count);
+    tab.per = (uint32_t)blocks;
+    uint32_t* ro = blocks == 1 ? (tab.nodes ? (uint32_t*)tab.roots : (uint32_t*)root_out) : nullptr;
     engine_switch(engine, [&](auto E) {
-      k_merkle_sub<decltype(E)::value><<<(unsigned)blocks, threads, 0, st>>>(
-          nodes, W, w, L, G1, blocks == 1 ? (uint32_t*)root_out : nullptr);
+      k_merkle_sub<decltype(E)::value><<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro, tab);
     });
     WHIR_HIP(hipGetLastError());
     w >>= L;
@@ -379,8 +415,14 @@ __device__ __forceinline__ void row_block(const uint64_t* __restrict__ cw, size_
 
 template <int D>
 __global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3(const uint64_t* __restrict__ cw, size_t N, size_t M,
-                                                                 uint8_t* __restrict__ nodes) {
-  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+                                                                 uint8_t* __restrict__ nodes, BatchTab tab) {
+  size_t blk = blockIdx.x;
+  if (tab.cw) {
+    cw = tab.cw[blk / tab.per];
+    nodes = tab.nodes[blk / tab.per];
+    blk %= tab.per;
+  }
+  const size_t g = blk * (size_t)blockDim.x + threadIdx.x;
   if (4 * g >= N) return;
   const size_t nb = M * D / 8;
   uint32_t left[8], par[8];
@@ -439,8 +481,14 @@ __device__ __forceinline__ void sel8(bool c, const uint32_t a[8], const uint32_t
 
 template <int D>
 __global__ __launch_bounds__(TREE_THREADS) void k_leaves_tree_b3w(const uint64_t* __restrict__ cw, size_t N, size_t M,
-                                                                  uint8_t* __restrict__ nodes) {
-  const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+                                                                  uint8_t* __restrict__ nodes, BatchTab tab) {
+  size_t blk = blockIdx.x;
+  if (tab.cw) {
+    cw = tab.cw[blk / tab.per];
+    nodes = tab.nodes[blk / tab.per];
+    blk %= tab.per;
+  }
+  const size_t wave = (blk * (size_t)blockDim.x + threadIdx.x) >> 6;
   const size_t R0 = wave * 256;
   if (R0 >= N) return;  // whole waves only (N is a multiple of 256)
   const int l = threadIdx.x & 63;
@@ -508,14 +556,18 @@ static bool fused_tree_ok(int engine, size_t N, size_t W) {
 }
 
 template <int D>
-static void commit_cols_t(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st) {
+static void commit_cols_t(int engine, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                          BatchTab tab = {}, size_t count = 1) {
   const size_t W = M * D;
+  tab.per = blocks_for(N / 4, TREE_THREADS);
+  const unsigned grid = tab.per * (unsigned)count;
+  const double bytes = (8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4)) * count;
   if (N >= 256) {
-    ProfScope prof(kernel_name("k_leaves_tree_b3w", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
-    k_leaves_tree_b3w<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+    ProfScope prof(kernel_name("k_leaves_tree_b3w", D), st, bytes);
+    k_leaves_tree_b3w<D><<<grid, TREE_THREADS, 0, st>>>(cw, N, M, nodes, tab);
   } else {
-    ProfScope prof(kernel_name("k_leaves_tree_b3", D), st, 8.0 * (double)W * N + 32.0 * (N + N / 2 + N / 4));
-    k_leaves_tree_b3<D><<<blocks_for(N / 4, TREE_THREADS), TREE_THREADS, 0, st>>>(cw, N, M, nodes);
+    ProfScope prof(kernel_name("k_leaves_tree_b3", D), st, bytes);
+    k_leaves_tree_b3<D><<<grid, TREE_THREADS, 0, st>>>(cw, N, M, nodes, tab);
   }
   WHIR_HIP(hipGetLastError());
 }
@@ -536,6 +588,34 @@ void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, 
     default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
   }
   merkle_from(engine, nodes, N, N / 4, st, root_out);
+}
+
+void commit_cols_many_dev(int engine, int D, const uint64_t* const* cw_tab, uint8_t* const* nodes_tab, size_t count,
+                          size_t N, size_t M, uint8_t* roots, hipStream_t st) {
+  WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D), "leaf hash: engine does not support the row size");
+  WHIR_REQUIRE(N >= 8 && (N & (N - 1)) == 0, "batched commit: codewords of 8 or more rows, a power of two");
+  if (!count) return;
+  BatchTab tab;
+  tab.cw = cw_tab;
+  tab.nodes = nodes_tab;
+  tab.roots = roots;
+  if (!fused_tree_ok(engine, N, M * D)) {
+    switch (D) {
+      case 1: leaves_cols<1>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+      case 2: leaves_cols<2>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+      case 3: leaves_cols<3>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+      default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
+    }
+    merkle_from(engine, nullptr, N, N, st, nullptr, tab, count);
+    return;
+  }
+  switch (D) {
+    case 1: commit_cols_t<1>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+    case 2: commit_cols_t<2>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+    case 3: commit_cols_t<3>(engine, nullptr, N, M, nullptr, st, tab, count); break;
+    default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
+  }
+  merkle_from(engine, nullptr, N, N / 4, st, nullptr, tab, count);
 }
 
 // ------------------------------------------------------------------ PoW

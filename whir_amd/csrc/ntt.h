@@ -11,6 +11,11 @@ namespace whir {
 // One interleaved encode: M messages, message j at msgs + j*msg_stride*D
 // (length L elements of degree D), optional masks (mask_len per message),
 // codeword length N = Q * 2^logC, written column-major to out[j*N + r].
+//
+// Several instances in one launch (batched commits): with msg_tab set, message j is
+// read from msg_tab[j / mpi] + (j % mpi) * msg_stride * D, and with out_tab set its
+// codeword goes to out_tab[j / mpi] + (j % mpi) * N * D (device arrays of one pointer
+// per instance, mpi messages each); msgs / out are then unused.
 struct RsArgs {
   const uint64_t* msgs;
   size_t msg_stride;
@@ -22,11 +27,19 @@ struct RsArgs {
   uint64_t Q;
   int logC;
   uint64_t M;
+  const uint64_t* const* msg_tab = nullptr;
+  uint64_t* const* out_tab = nullptr;
+  uint64_t mpi = 1;
 };
 
 // Words of device scratch rs_encode_cols needs for these sizes.
 size_t rs_scratch_words(int D, uint64_t M, uint64_t N, uint64_t C);
 void rs_encode_cols(int D, RsArgs a, uint64_t* scratch, hipStream_t st);
+// While set on the calling thread (a stream capture is in progress), rs_encode_cols
+// neither tunes nor builds twiddle tables: it uses recorded or built-in schedules and
+// requires the tables of its shapes to exist (rs_prepare builds them beforehand).
+void rs_set_capturing(bool on);
+void rs_prepare(int D, uint64_t N, int logC, uint64_t M, hipStream_t st);
 void cols_to_rows(int D, const uint64_t* cw, uint64_t* out, size_t N, size_t M, hipStream_t st);
 
 // Schedule autotuner of rs_encode_cols (see ntt.hip): tune new shapes on first use,

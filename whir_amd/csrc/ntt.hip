@@ -41,6 +41,9 @@ using gl::Ext;
 // Threads per block: pass 1 runs 256 (its per-thread post factors and message loop
 // favour fewer, fuller threads); pass 2 and the single pass run 512, which holds
 // twice the waves per LDS tile (measured 5-10% faster on MI355X than 256).
+#ifndef NTT_REM_FIRST
+#define NTT_REM_FIRST 1
+#endif
 static constexpr int P1_THREADS = 256;
 static constexpr int NTT_THREADS = 512;
 static constexpr int MAX_PASS_LOG = 11;   // largest DFT staged in one pass (a 2^11 Fp3
@@ -103,6 +106,7 @@ TwTable twiddles(int logn) {
   auto key = std::make_pair(dev, logn);
   auto it = g_tw.find(key);
   if (it != g_tw.end()) return it->second;
+  WHIR_REQUIRE(!capture_active(), "ntt: twiddle table built during a stream capture");
   int h = (logn + 1) / 2;
   uint64_t w = gl::root_of_unity(1ull << logn);
   std::vector<uint64_t> lo(1ull << h), hi(1ull << (logn - h));
@@ -126,6 +130,7 @@ static const uint64_t* flat_twiddles(int logS) {
   auto key = std::make_pair(dev, logS);
   auto it = g_flat.find(key);
   if (it != g_flat.end()) return it->second;
+  WHIR_REQUIRE(!capture_active(), "ntt: twiddle table built during a stream capture");
   std::vector<uint64_t> v((size_t)1 << logS);
   uint64_t w = gl::root_of_unity(1ull << logS);
   uint64_t x = 1;
@@ -170,6 +175,7 @@ static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipS
     auto it = g_shape.find({dev, kind, logN, logC, logC1});
     if (it != g_shape.end()) return it->second;
   }
+  WHIR_REQUIRE(!capture_active(), "ntt: shape table built during a stream capture");
   TwTable twN = twiddles(logN);
   uint64_t N = 1ull << logN;
   uint64_t len = kind == 2 ? (N >> logC) << (logC - logC1) : N;
@@ -195,14 +201,29 @@ static const uint64_t* shape_table(int kind, int logN, int logC, int logC1, hipS
 // writes it back in place; that is exactly the state after log2(R) radix-2 DIF
 // stages, so the steps compose to the plain bit-reversed DIF.
 // --------------------------------------------------------------------------
-// LDS tiles are padded by one element per 32 (element e lives at e + e/32): the
+// Fp3 LDS tiles are padded by one element per 32 (element e lives at e + e/32): the
 // strided butterfly accesses, the transposed tile writes and the bit-reversed reads
-// then spread over the banks instead of piling onto a few (up to 64-way conflicts
-// unpadded, about the minimum for the access width padded).
-template <bool PAD>
-__device__ __forceinline__ uint32_t lidx(uint32_t e) { return PAD ? e + (e >> 5) : e; }
-template <bool PAD>
-__host__ __device__ __forceinline__ size_t padded(size_t n) { return PAD ? n + (n >> 5) : n; }
+// then spread over the banks instead of piling onto a few.
+//
+// With NTT_SWZ, Fp and Fp2 tiles are not padded but XOR-swizzled instead: within each
+// aligned run of 16 (Fp, 8-byte elements) or 8 (Fp2, 16-byte elements) elements, the
+// slot is XORed with higher index bits, so that any 16 (Fp) or 8 (Fp2) lanes that
+// access elements a power-of-two stride apart (the transposed tile loads and stores,
+// the 8-consecutive-element groups of a last radix-8 level) land in distinct banks.
+#ifndef NTT_SWZ
+#define NTT_SWZ 1
+#endif
+template <int D, bool PAD>
+__device__ __forceinline__ uint32_t lidx(uint32_t e) {
+  if constexpr (NTT_SWZ && D == 2) return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9)) & 7u);
+  if constexpr (NTT_SWZ && D == 1) return e ^ (((e >> 4) ^ (e >> 8)) & 15u);
+  return PAD ? e + (e >> 5) : e;
+}
+template <int D, bool PAD>
+__host__ __device__ __forceinline__ size_t padded(size_t n) {
+  if constexpr (NTT_SWZ && D <= 2) return n;
+  return PAD ? n + (n >> 5) : n;
+}
 
 template <int D>
 __device__ __forceinline__ Ext<D> rot(const Ext<D>& a, unsigned k) {
@@ -317,12 +338,12 @@ __device__ __forceinline__ void dif_step(Ext<D>* s, const uint64_t* tw, int logS
     const uint32_t e0 = ((uint32_t)blk << logSub) + base;
     Ext<D> y[R];
 #pragma unroll
-    for (int m = 0; m < R; m++) y[m] = s[lidx<PAD>(e0 + m * stride)];
+    for (int m = 0; m < R; m++) y[m] = s[lidx<D, PAD>(e0 + m * stride)];
     dif_regs<D, R>(y);
 #pragma unroll
     for (int m = 0; m < R; m++) {
       const int e = (base * brev_small<R>(m)) << tw_shift;
-      s[lidx<PAD>(e0 + m * stride)] = (m == 0 || !TW) ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
+      s[lidx<D, PAD>(e0 + m * stride)] = (m == 0 || !TW) ? y[m] : gl::escale<D>(y[m], tw[e & (S - 1)]);
     }
   }
   (void)S;
@@ -342,6 +363,22 @@ __device__ __forceinline__ void dif_level(Ext<D>* s, const uint64_t* tw, int log
 template <int D, bool PAD, bool R16>
 __device__ __forceinline__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS, int cols) {
   int logSub = logS;
+#if NTT_REM_FIRST
+  if (!R16 && logSub > 3 && logSub % 3) {
+    // The levels that do not fill a radix-8 step go first, with their twiddles (half or
+    // three quarters of a multiply per element), so that the last level is a whole
+    // radix-8 step, whose factors are all 1 (instead of a radix-8 step with 7/8 of a
+    // multiply and a twiddle-free radix-2 or radix-4 step at the end).
+    if (logSub % 3 == 1) {
+      dif_step<D, PAD, 2, 1, true>(s, tw, logS, logSub, cols);
+      logSub -= 1;
+    } else {
+      dif_step<D, PAD, 4, 2, true>(s, tw, logS, logSub, cols);
+      logSub -= 2;
+    }
+    __syncthreads();
+  }
+#endif
   if (R16) {
     while (logSub >= 4) {
       dif_level<D, PAD, 16, 4>(s, tw, logS, logSub, cols);
@@ -394,6 +431,16 @@ __device__ __forceinline__ Ext<D> coeff_at(const uint64_t* msg, size_t L, const 
   return r;
 }
 
+// Message j's coefficients and codeword (see RsArgs).
+template <int D>
+__device__ __forceinline__ const uint64_t* msg_ptr(const RsArgs& a, size_t j) {
+  return a.msg_tab ? a.msg_tab[j / a.mpi] + (j % a.mpi) * a.msg_stride * D : a.msgs + j * a.msg_stride * D;
+}
+template <int D>
+__device__ __forceinline__ uint64_t* out_ptr(const RsArgs& a, size_t j) {
+  return a.out_tab ? a.out_tab[j / a.mpi] + (j % a.mpi) * a.N * D : a.out + j * a.N * D;
+}
+
 // --------------------------------------------------------------------------
 // Single pass: C = 2^logC <= 2^11.  Each block transforms G sequences.
 // pre[c*C + i] = w_N^(c i).
@@ -405,7 +452,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
   const int logC = a.logC;
   const int C = 1 << logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + padded<PAD>((size_t)G * C) * D;
+  uint64_t* tws = smem + padded<D, PAD>((size_t)G * C) * D;
   for (int k = threadIdx.x; k < C; k += blockDim.x) tws[k] = twS[k];
   const size_t nseq = (size_t)a.M * a.Q;
   const size_t seq0 = (size_t)blockIdx.x * G;
@@ -415,11 +462,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
     Ext<D> v = gl::ext_zero<D>();
     if (seq < nseq) {
       size_t j = seq / a.Q, c = seq % a.Q;
-      v = coeff_at<D>(a.msgs + j * a.msg_stride * D, a.L, a.masks ? a.masks + j * a.mask_len * D : nullptr,
-                      a.mask_len, i);
+      v = coeff_at<D>(msg_ptr<D>(a, j), a.L, a.masks ? a.masks + j * a.mask_len * D : nullptr, a.mask_len, i);
       if (c) v = gl::escale<D>(v, pre[c * C + i]);
     }
-    s[lidx<PAD>(e)] = v;
+    s[lidx<D, PAD>(e)] = v;
   }
   __syncthreads();
   lds_dft<D, PAD, R16>(s, tws, logC, G);
@@ -428,8 +474,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_single(RsArgs a, const uint
     size_t seq = seq0 + g;
     if (seq >= nseq) continue;
     size_t j = seq / a.Q, c = seq % a.Q;
-    Ext<D> v = s[lidx<PAD>(((uint32_t)g << logC) + bitrev(t, logC))];
-    uint64_t* o = a.out + (j * a.N + c * C + t) * D;
+    Ext<D> v = s[lidx<D, PAD>(((uint32_t)g << logC) + bitrev(t, logC))];
+    uint64_t* o = out_ptr<D>(a, j) + (c * C + t) * D;
 #pragma unroll
     for (int d = 0; d < D; d++) o[d] = v.c[d];
   }
@@ -454,7 +500,7 @@ __global__ __launch_bounds__(P1_THREADS) void k_ntt_pass1(RsArgs a, const uint64
   const size_t C = (size_t)1 << a.logC;
   const int tile = C2 << logb;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + padded<PAD>((size_t)tile) * D;
+  uint64_t* tws = smem + padded<D, PAD>((size_t)tile) * D;
   for (int k = threadIdx.x; k < C2; k += blockDim.x) tws[k] = twS[k];
   const uint32_t ntile = (uint32_t)(C1 >> logb);
   const uint32_t njg = (uint32_t)((a.M + jn - 1) / jn);
@@ -476,13 +522,13 @@ __global__ __launch_bounds__(P1_THREADS) void k_ntt_pass1(RsArgs a, const uint64
     }
   }
   for (size_t j = j0; j < j1; j++) {
-    const uint64_t* msg = a.msgs + j * a.msg_stride * D;
+    const uint64_t* msg = msg_ptr<D>(a, j);
     const uint64_t* mask = a.masks ? a.masks + j * a.mask_len * D : nullptr;
     for (int e = threadIdx.x; e < tile; e += blockDim.x) {
       int col = e & (b - 1), i2 = e >> logb;
       Ext<D> v = coeff_at<D>(msg, a.L, mask, a.mask_len, (size_t)(i1_0 + col) + ((size_t)i2 << logC1));
       if (c) v = gl::escale<D>(v, rowc[i2]);
-      s[lidx<PAD>(((uint32_t)col << logC2) + i2)] = v;
+      s[lidx<D, PAD>(((uint32_t)col << logC2) + i2)] = v;
     }
     __syncthreads();
     lds_dft<D, PAD, R16>(s, tws, logC2, b);
@@ -494,7 +540,7 @@ __global__ __launch_bounds__(P1_THREADS) void k_ntt_pass1(RsArgs a, const uint64
         if (e < tile) {
           int col = e & (b - 1), t2 = e >> logb, li = bitrev(t2, logC2);
           size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
-          Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], pf[k]);
+          Ext<D> v = gl::escale<D>(s[lidx<D, PAD>(((uint32_t)col << logC2) + li)], pf[k]);
           uint64_t* o = yj + o_idx * D;
 #pragma unroll
           for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -504,7 +550,7 @@ __global__ __launch_bounds__(P1_THREADS) void k_ntt_pass1(RsArgs a, const uint64
       for (int e = threadIdx.x; e < tile; e += blockDim.x) {
         int col = e & (b - 1), t2 = e >> logb, li = bitrev(t2, logC2);
         size_t o_idx = ((size_t)t2 << logC1) + i1_0 + col;
-        Ext<D> v = gl::escale<D>(s[lidx<PAD>(((uint32_t)col << logC2) + li)], postc[o_idx]);
+        Ext<D> v = gl::escale<D>(s[lidx<D, PAD>(((uint32_t)col << logC2) + li)], postc[o_idx]);
         uint64_t* o = yj + o_idx * D;
 #pragma unroll
         for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -531,7 +577,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
   const int C1 = 1 << logC1, C2 = 1 << logC2, b = 1 << logb;
   const size_t C = (size_t)1 << a.logC;
   Ext<D>* s = (Ext<D>*)smem;
-  uint64_t* tws = smem + padded<PAD>((size_t)C1 << logb) * D;
+  uint64_t* tws = smem + padded<D, PAD>((size_t)C1 << logb) * D;
   for (int k = threadIdx.x; k < C1; k += blockDim.x) tws[k] = twS[k];
   // logical block = (j * Q + c) * ntile + tile: neighbouring row tiles are adjacent
   const uint32_t ntile = (uint32_t)(C2 >> logb);
@@ -547,14 +593,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass2(RsArgs a, const uint6
     Ext<D> v;
 #pragma unroll
     for (int d = 0; d < D; d++) v.c[d] = src[d];
-    s[lidx<PAD>(e)] = v;
+    s[lidx<D, PAD>(e)] = v;
   }
   __syncthreads();
   lds_dft<D, PAD, R16>(s, tws, logC1, b);
-  uint64_t* oj = a.out + (j * a.N + c * C) * D;
+  uint64_t* oj = out_ptr<D>(a, j) + c * C * D;
   for (int e = threadIdx.x; e < tile; e += blockDim.x) {
     int row = e & (b - 1), t1 = e >> logb, li = bitrev(t1, logC1);
-    Ext<D> v = s[lidx<PAD>(((uint32_t)row << logC1) + li)];
+    Ext<D> v = s[lidx<D, PAD>(((uint32_t)row << logC1) + li)];
     uint64_t* o = oj + ((size_t)(t2_0 + row) + ((size_t)t1 << logC2)) * D;
 #pragma unroll
     for (int d = 0; d < D; d++) o[d] = v.c[d];
@@ -658,7 +704,7 @@ static void pass1_t(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN,
   const uint64_t base_blocks = (uint64_t)(C1 >> logb) * a.Q;
   int jn = 1;
   while (jn * 2 <= (int)a.M && base_blocks * ((a.M + jn * 2 - 1) / (jn * 2)) >= 2048) jn *= 2;
-  size_t smem = (padded<PAD>((size_t)tile) * D + C2) * 8;
+  size_t smem = (padded<D, PAD>((size_t)tile) * D + C2) * 8;
   dim3 grid(xcd_grid(base_blocks * ((a.M + jn - 1) / jn)));
   allow_lds(k_ntt_pass1<D, EPT, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_pass1", D, EPT), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
@@ -681,7 +727,7 @@ static void pass1(RsArgs a, const uint64_t* post, uint64_t* scratch, int logN, i
 template <int D, bool PAD, bool R16>
 static void single_t(RsArgs a, const uint64_t* pre, int G, size_t nseq, hipStream_t st) {
   const int C = 1 << a.logC;
-  size_t smem = (padded<PAD>((size_t)G * C) * D + C) * 8;
+  size_t smem = (padded<D, PAD>((size_t)G * C) * D + C) * 8;
   unsigned blocks = (unsigned)((nseq + G - 1) / G);
   allow_lds(k_ntt_single<D, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_single", D), st, 8.0 * D * ((double)a.M * a.L + (double)a.M * a.N));
@@ -709,7 +755,7 @@ static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, int tw, hipStream_t 
   const int logC2 = a.logC - logC1;
   const int C1 = 1 << logC1, C2 = 1 << logC2;
   int logb = std::min(logC2, floor_log2(std::max(1, tw / (C1 * D))));
-  size_t smem = (padded<PAD>((size_t)C1 << logb) * D + C1) * 8;
+  size_t smem = (padded<D, PAD>((size_t)C1 << logb) * D + C1) * 8;
   dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
   allow_lds(k_ntt_pass2<D, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
@@ -731,7 +777,8 @@ static void pass2(RsArgs a, uint64_t* scratch, int logC1, int v, int tw, hipStre
 // With tuning on (whir_ntt_autotune(1) or WHIR_NTT_AUTOTUNE=1), the first encode of
 // each shape (degree, log N, log C, messages) times every candidate schedule on the
 // caller's stream, with the caller's data, and keeps the fastest:
-//   the split log C1 in {ceil(logC/2), floor(logC/2)} (two-pass shapes),
+//   the split log C1 in {ceil(logC/2), floor(logC/2)} and one level either side of
+//   them (two-pass shapes),
 //   per pass, the variant (padding x radix 16) and the tile size (4096 or 8192
 //   words, where the device's LDS allows it),
 // each pass tuned on its own (pass 1, then pass 2 on pass 1's output), each
@@ -826,8 +873,11 @@ static SchedEntry tune(RsArgs a, uint64_t* scratch, hipStream_t st) {
   }
   constexpr int EPT = pass1_ept<D>();
   double best_total = 1e30;
+  // the balanced split(s) first, then one level either side of them
   std::vector<int> splits{def.logC1};
   if (logC % 2) splits.push_back(logC / 2);
+  for (int d : {def.logC1 + 1, logC / 2 - 1})
+    if (std::find(splits.begin(), splits.end(), d) == splits.end()) splits.push_back(d);
   for (int logC1 : splits) {
     const int logC2 = logC - logC1;
     if (logC1 > MAX_PASS_LOG || logC2 > MAX_PASS_LOG) continue;
@@ -894,21 +944,21 @@ static void run_sched(RsArgs a, uint64_t* scratch, const NttSched& s, hipStream_
   pass2<D>(a, scratch, s.logC1, s.v2, s.tw2, st);
 }
 
+// The schedule recorded for a shape, else the built-in one.
+static NttSched sched_for(int D, uint64_t N, int logC, uint64_t M, bool* found = nullptr) {
+  const SchedKey key{D, log2_exact(N), logC, M};
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  auto it = g_sched.find(key);
+  if (found) *found = it != g_sched.end();
+  return it != g_sched.end() ? it->second.s : default_sched(D, logC);
+}
+
 template <int D>
 static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
   WHIR_REQUIRE(a.logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
-  const SchedKey key{D, log2_exact(a.N), a.logC, a.M};
-  NttSched s;
   bool found = false;
-  {
-    std::lock_guard<std::mutex> g(g_sched_mu);
-    auto it = g_sched.find(key);
-    if (it != g_sched.end()) {
-      s = it->second.s;
-      found = true;
-    }
-  }
-  if (found) {
+  const NttSched s = sched_for(D, a.N, a.logC, a.M, &found);
+  if (found || capture_active()) {  // no tuning inside a stream capture
     run_sched<D>(a, scratch, s, st);
     return;
   }
@@ -918,7 +968,25 @@ static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
   }
   SchedEntry e = tune<D>(a, scratch, st);  // also leaves the output behind
   std::lock_guard<std::mutex> g(g_sched_mu);
-  g_sched.emplace(key, e);
+  g_sched.emplace(SchedKey{D, log2_exact(a.N), a.logC, a.M}, e);
+}
+
+void rs_set_capturing(bool on) { set_capture_active(on); }
+
+void rs_prepare(int D, uint64_t N, int logC, uint64_t M, hipStream_t st) {
+  WHIR_REQUIRE(D >= 1 && D <= 3, "rs_prepare: extension degree must be 1, 2 or 3");
+  const int logN = log2_exact(N);
+  WHIR_REQUIRE(logN >= 0 && logC <= logN, "rs_prepare: bad shape");
+  const NttSched s = sched_for(D, N, logC, M);
+  if (logC <= MAX_PASS_LOG) {
+    (void)shape_table(0, logN, logC, 0, st);
+    (void)flat_twiddles(logC);
+  } else {
+    (void)shape_table(1, logN, logC, s.logC1, st);
+    (void)shape_table(2, logN, logC, s.logC1, st);
+    (void)flat_twiddles(logC - s.logC1);
+    (void)flat_twiddles(s.logC1);
+  }
 }
 
 size_t rs_scratch_words(int D, uint64_t M, uint64_t N, uint64_t C) {

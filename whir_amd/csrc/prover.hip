@@ -532,6 +532,232 @@ void commit_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs,
   irs_commit<D>(t, cfg.initial, cfg.engine, vecs, nvec, 1, w, s);
 }
 
+// ------------------------------------------------------------------ batched commits
+// whir::Config::commit (irs_commit.rs:292-350) of many independent instances, each with
+// its own configuration, transcript and vector, as shared launches: the instances are
+// grouped by shape (message length, codeword length, messages, hash engine), and each
+// group's encodes, leaf hashes and Merkle levels run as one launch per step over
+// device tables of the instances' vectors, codewords and trees.  That launch sequence
+// depends only on the batch's layout (the list of shapes), so it is recorded once as a
+// HIP graph (stream capture) and replayed for every later batch of the same layout,
+// with the tables rewritten before each replay.  Then, on the host, each root goes to
+// its transcript and the OOD points come out; one batched launch evaluates every
+// instance's vector at its points, and the values go to the transcripts.  Every
+// transcript sees the calls of a plain commit, in the same order, so the witnesses
+// and transcripts are those of commit_impl.
+struct CommitShape {
+  int engine;
+  size_t L, N, M;
+  bool operator==(const CommitShape& o) const {
+    return engine == o.engine && L == o.L && N == o.N && M == o.M;
+  }
+};
+
+struct CommitGroup {
+  CommitShape shape;
+  std::vector<size_t> members;  // positions in the batch
+  DevBuf tabs;                  // device pointer tables: [messages][codewords][nodes]
+  size_t root0 = 0;             // first root slot of the group
+};
+
+struct CommitPlan {
+  std::vector<CommitShape> layout;
+  std::vector<CommitGroup> groups;
+  DevBuf roots;    // 32 bytes per batched instance
+  DevBuf scratch;  // NTT scratch of the largest group (the groups run one after another)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t uses = 0;
+  ~CommitPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+// Plans of the calling thread's stream, most recently used first (a few layouts).
+thread_local std::vector<std::unique_ptr<CommitPlan>> t_plans;
+constexpr size_t MAX_PLANS = 4;
+
+}  // namespace
+
+// Frees the calling thread's plans (their buffers go back to the thread's stream, their
+// graphs are destroyed); called before the thread's stream is released.
+void release_commit_plans() { t_plans.clear(); }
+
+namespace {
+
+bool batchable(const WhirCfg& c) {
+  const IrsCfg& irs = c.initial;
+  const size_t L = irs.message_length(), N = irs.codeword_length;
+  return irs.num_vectors == 1 && irs.mask_length == 0 && L > 0 && (L & (L - 1)) == 0 && N >= 8 &&
+         (N & (N - 1)) == 0 && N % L == 0;
+}
+
+void run_commit_plan(CommitPlan& P, hipStream_t s) {
+  for (CommitGroup& g : P.groups) {
+    const CommitShape& sh = g.shape;
+    const size_t cnt = g.members.size();
+    const uint64_t* const* msg_tab = (const uint64_t* const*)g.tabs.ptr;
+    uint64_t* const* cw_tab = (uint64_t* const*)(g.tabs.as<uint64_t*>() + cnt);
+    uint8_t* const* nodes_tab = (uint8_t* const*)(g.tabs.as<uint64_t*>() + 2 * cnt);
+    RsArgs a{nullptr, sh.L, sh.L, nullptr, 0, nullptr, sh.N, sh.N / sh.L, log2_exact(sh.L), sh.M * cnt};
+    a.msg_tab = msg_tab;
+    a.out_tab = cw_tab;
+    a.mpi = sh.M;
+    rs_encode_cols(1, a, P.scratch.as<uint64_t>(), s);
+    commit_cols_many_dev(sh.engine, 1, (const uint64_t* const*)cw_tab, nodes_tab, cnt, sh.N, sh.M,
+                         P.roots.as<uint8_t>() + 32 * g.root0, s);
+  }
+}
+
+CommitPlan& commit_plan(const std::vector<CommitShape>& layout, hipStream_t s) {
+  for (size_t i = 0; i < t_plans.size(); i++)
+    if (t_plans[i]->layout == layout) {
+      if (i) std::rotate(t_plans.begin(), t_plans.begin() + i, t_plans.begin() + i + 1);
+      return *t_plans[0];
+    }
+  auto P = std::make_unique<CommitPlan>();
+  P->layout = layout;
+  for (size_t k = 0; k < layout.size(); k++) {
+    auto it = std::find_if(P->groups.begin(), P->groups.end(), [&](const CommitGroup& g) { return g.shape == layout[k]; });
+    if (it == P->groups.end()) {
+      P->groups.emplace_back();
+      P->groups.back().shape = layout[k];
+      it = P->groups.end() - 1;
+    }
+    it->members.push_back(k);
+  }
+  size_t slot = 0, scratch = 0;
+  for (CommitGroup& g : P->groups) {
+    const size_t cnt = g.members.size();
+    g.root0 = slot;
+    slot += cnt;
+    g.tabs = DevBuf(3 * cnt * 8, s);
+    scratch = std::max(scratch, rs_scratch_words(1, g.shape.M * cnt, g.shape.N, g.shape.L));
+  }
+  P->roots = DevBuf(32 * layout.size(), s);
+  P->scratch = DevBuf(scratch * 8, s);
+  if (t_plans.size() >= MAX_PLANS) t_plans.pop_back();
+  t_plans.insert(t_plans.begin(), std::move(P));
+  return *t_plans[0];
+}
+
+template <int D>
+void finish_ood(const std::vector<const WhirCfg*>& cfgs, std::vector<Transcript*>& ts,
+                const std::vector<const uint64_t*>& vecs, std::vector<DevWitness*>& ws, const std::vector<size_t>& idx,
+                hipStream_t s) {
+  std::vector<UniJob> jobs;
+  std::vector<std::pair<size_t, size_t>> where;  // (instance, sample)
+  for (size_t k : idx) {
+    const IrsCfg& c = cfgs[k]->initial;
+    ws[k]->ood_points.clear();
+    for (size_t i = 0; i < c.out_domain_samples; i++) {
+      const E<D> x = verifier_field<D>(*ts[k]);
+      push<D>(ws[k]->ood_points, x);
+      UniJob j{vecs[k], c.vector_size, {0, 0, 0}};
+      for (int d = 0; d < D; d++) j.x[d] = x.c[d];
+      jobs.push_back(j);
+      where.emplace_back(k, i);
+    }
+  }
+  std::vector<uint64_t> vals(jobs.size() * D);
+  if (!jobs.empty()) {
+    DevBuf out(jobs.size() * D * 8, s);
+    univariate_eval_batch_dev(D, jobs.data(), jobs.size(), out.as<uint64_t>(), s);
+    download_sync(vals.data(), out.ptr, vals.size() * 8, s);
+  }
+  for (size_t q = 0; q < jobs.size(); q++) {
+    DevWitness& w = *ws[where[q].first];
+    w.ood_values.insert(w.ood_values.end(), &vals[q * D], &vals[(q + 1) * D]);
+    ts[where[q].first]->prover_field(D, &vals[q * D], 1);
+  }
+}
+
+void commit_many(const std::vector<const WhirCfg*>& cfgs, std::vector<Transcript*>& ts,
+                 const std::vector<const uint64_t*>& vecs, std::vector<DevWitness*>& ws, hipStream_t s) {
+  const size_t n = cfgs.size();
+  // Instances outside the batched path commit one by one.
+  std::vector<size_t> batched;
+  for (size_t k = 0; k < n; k++) {
+    if (batchable(*cfgs[k])) {
+      batched.push_back(k);
+      continue;
+    }
+    switch (cfgs[k]->D) {
+      case 1: commit_impl<1>(*cfgs[k], *ts[k], &vecs[k], 1, *ws[k], s); break;
+      case 2: commit_impl<2>(*cfgs[k], *ts[k], &vecs[k], 1, *ws[k], s); break;
+      case 3: commit_impl<3>(*cfgs[k], *ts[k], &vecs[k], 1, *ws[k], s); break;
+    }
+  }
+  if (batched.empty()) return;
+  std::vector<CommitShape> layout;
+  for (size_t k : batched) {
+    const IrsCfg& c = cfgs[k]->initial;
+    layout.push_back({cfgs[k]->engine, c.message_length(), c.codeword_length, c.num_messages()});
+  }
+  CommitPlan& P = commit_plan(layout, s);
+  // This batch's witnesses and the tables that point the plan's launches at them.
+  for (CommitGroup& g : P.groups) {
+    const size_t cnt = g.members.size();
+    std::vector<uint64_t> tab(3 * cnt);
+    for (size_t i = 0; i < cnt; i++) {
+      const size_t k = batched[g.members[i]];
+      DevWitness& w = *ws[k];
+      w.Ds = 1;
+      w.N = g.shape.N;
+      w.M = g.shape.M;
+      w.num_vectors = 1;
+      w.cw = DevBuf(g.shape.M * g.shape.N * 8, s);
+      w.nodes = DevBuf(32 * merkle_num_nodes(g.shape.N), s);
+      tab[i] = (uint64_t)(uintptr_t)vecs[k];
+      tab[cnt + i] = (uint64_t)(uintptr_t)w.cw.ptr;
+      tab[2 * cnt + i] = (uint64_t)(uintptr_t)w.nodes.ptr;
+    }
+    upload_async(g.tabs.ptr, tab.data(), tab.size() * 8, s);
+  }
+  if (P.exec) {
+    WHIR_HIP(hipGraphLaunch(P.exec, s));
+  } else {
+    // First batch of this layout: run the launches (which also builds the twiddle
+    // tables and tunes new NTT shapes), then record them for the later batches.
+    run_commit_plan(P, s);
+    WHIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    set_capture_active(true);
+    try {
+      run_commit_plan(P, s);
+    } catch (...) {
+      set_capture_active(false);
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    set_capture_active(false);
+    WHIR_HIP(hipStreamEndCapture(s, &P.graph));
+    WHIR_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+  }
+  P.uses++;
+  std::vector<uint8_t> roots(32 * batched.size());
+  download_sync(roots.data(), P.roots.ptr, roots.size(), s);
+  for (const CommitGroup& g : P.groups)
+    for (size_t i = 0; i < g.members.size(); i++) {
+      const size_t k = batched[g.members[i]];
+      ts[k]->prover_hash(&roots[32 * (g.root0 + i)]);
+    }
+  // OOD samples of every batched instance, grouped by extension degree.
+  for (int D = 1; D <= 3; D++) {
+    std::vector<size_t> idx;
+    for (size_t k : batched)
+      if (cfgs[k]->D == D) idx.push_back(k);
+    if (idx.empty()) continue;
+    switch (D) {
+      case 1: finish_ood<1>(cfgs, ts, vecs, ws, idx, s); break;
+      case 2: finish_ood<2>(cfgs, ts, vecs, ws, idx, s); break;
+      case 3: finish_ood<3>(cfgs, ts, vecs, ws, idx, s); break;
+    }
+  }
+}
+
 // whir::Config::prove, src/protocols/whir/prover.rs:50-316: vecs[0..V) are the
 // committed base-field vectors, witnesses ws[0..W) hold them batch_size at a time
 // in order (V = W * batch_size).  Linear form k is a MultilinearExtension (kinds
@@ -869,6 +1095,34 @@ int whir_commit_batch_dev(const whir_config* c, whir_transcript* t, const uint64
 
 int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, whir_witness** out) {
   return whir_commit_batch_dev(c, t, &vector, 1, out);
+}
+
+int whir_commit_many_dev(const whir_config* const* cfgs, whir_transcript* const* ts, const uint64_t* const* vectors,
+                         size_t count, whir_witness** out) {
+  API_BEGIN
+  WHIR_REQUIRE(count == 0 || (cfgs && ts && vectors && out), "commit_many: null argument");
+  std::vector<const WhirCfg*> cs(count);
+  std::vector<Transcript*> tr(count);
+  std::vector<const uint64_t*> vs(count);
+  for (size_t k = 0; k < count; k++) {
+    WHIR_REQUIRE(cfgs[k] && ts[k] && vectors[k], "commit_many: null configuration, transcript or vector");
+    WHIR_REQUIRE(cfgs[k]->cfg.initial.num_vectors == 1, "commit_many: configurations of batch_size 1 only");
+    for (size_t j = 0; j < k; j++)
+      WHIR_REQUIRE(ts[j] != ts[k], "commit_many: each instance needs its own transcript");
+    cs[k] = &cfgs[k]->cfg;
+    tr[k] = ts[k]->t.get();
+    vs[k] = vectors[k];
+    out[k] = nullptr;
+  }
+  std::vector<std::unique_ptr<whir_witness>> ws(count);
+  std::vector<DevWitness*> wp(count);
+  for (size_t k = 0; k < count; k++) {
+    ws[k] = std::make_unique<whir_witness>();
+    wp[k] = &ws[k]->w;
+  }
+  commit_many(cs, tr, vs, wp, thread_stream());
+  for (size_t k = 0; k < count; k++) out[k] = ws[k].release();
+  API_END
 }
 
 int whir_prove_forms_dev(const whir_config* c, whir_transcript* t, const uint64_t* const* vectors, size_t nvec,

@@ -242,6 +242,7 @@ thread_local Readback t_readback;
 }  // namespace
 
 void release_thread_resources() {
+  release_commit_plans();
   t_stream.release();
   t_staging.release();
   t_readback.release();
@@ -301,6 +302,10 @@ hipEvent_t take_event() {
 
 bool profile_on() { return g_prof_on; }
 
+static thread_local bool t_capture = false;
+bool capture_active() { return t_capture; }
+void set_capture_active(bool on) { t_capture = on; }
+
 const char* kernel_name(const char* base, int a, int b) {
   static std::mutex mu;
   static std::map<std::tuple<std::string, int, int>, std::string> names;
@@ -315,7 +320,7 @@ const char* kernel_name(const char* base, int a, int b) {
 }
 
 ProfScope::ProfScope(const char* n, hipStream_t st, double b) : name(n), s(st), bytes(b) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || t_capture) return;
   start = take_event();
   if (start) (void)hipEventRecord(start, s);
 }

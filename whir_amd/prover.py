@@ -200,6 +200,19 @@ def commit(cfg: Config, t: Transcript, d_vector: DeviceBuffer) -> Witness:
     return Witness(h.value)
 
 
+def commit_many(cfgs, transcripts, d_vectors):
+    """whir::Config::commit of many independent instances in shared launches (a HIP
+    graph per batch layout); the same witnesses and transcripts as commit() on each."""
+    n = len(cfgs)
+    assert len(transcripts) == n and len(d_vectors) == n
+    c = (vp * max(n, 1))(*[x.h for x in cfgs])
+    t = (vp * max(n, 1))(*[x.h for x in transcripts])
+    v = (vp * max(n, 1))(*[x.ptr for x in d_vectors])
+    out = (vp * max(n, 1))()
+    _call("whir_commit_many_dev", c, t, v, n, out)
+    return [Witness(out[k]) for k in range(n)]
+
+
 def prove(cfg: Config, t: Transcript, d_vector: DeviceBuffer, w: Witness, points, evaluations) -> np.ndarray:
     """points: (num_points, num_variables * degree) words; evaluations: (num_points, degree)."""
     D, nv = cfg.degree, cfg.num_variables
@@ -349,6 +362,24 @@ class Executor:
         _call("whir_executor_submit_batch", self.h, cfg.h, t.h, ptrs, len(d_vectors), _p64(pts), npts, _p64(ev),
               ctypes.byref(jid))
         self._keep[jid.value] = (cfg, t, list(d_vectors), self.COMMIT_AND_PROVE)
+        return jid.value
+
+    def submit_prove(self, cfg: Config, t: Transcript, d_vector: DeviceBuffer, witness: Witness, points=None,
+                     evaluations=None) -> int:
+        """Prove only, for a vector already committed to in t (witness from commit or
+        commit_many)."""
+        D, nv = cfg.degree, cfg.num_variables
+        pts = np.ascontiguousarray(np.asarray(points if points is not None else [], dtype=np.uint64)).reshape(-1)
+        ev = np.ascontiguousarray(np.asarray(evaluations if evaluations is not None else [],
+                                             dtype=np.uint64)).reshape(-1)
+        npts = ev.size // D
+        assert pts.size == npts * nv * D, "points must be num_points x num_variables x degree words"
+        pts = pts if pts.size else np.zeros(1, dtype=np.uint64)
+        ev = ev if ev.size else np.zeros(1, dtype=np.uint64)
+        jid = ctypes.c_uint64()
+        _call("whir_executor_submit_prove", self.h, cfg.h, t.h, d_vector.ptr, witness.h, _p64(pts), npts, _p64(ev),
+              ctypes.byref(jid))
+        self._keep[jid.value] = (cfg, t, (d_vector, witness), self.COMMIT_AND_PROVE)
         return jid.value
 
     def wait(self, job_id: int):
