@@ -557,22 +557,48 @@ struct CommitGroup {
   CommitShape shape;
   std::vector<size_t> members;  // positions in the batch
   DevBuf tabs;                  // device pointer tables: [messages][codewords][nodes]
+  DevBuf scratch;               // the group's NTT scratch (groups run concurrently)
   size_t root0 = 0;             // first root slot of the group
+  hipEvent_t done = nullptr;
 };
 
 struct CommitPlan {
   std::vector<CommitShape> layout;
   std::vector<CommitGroup> groups;
-  DevBuf roots;    // 32 bytes per batched instance
-  DevBuf scratch;  // NTT scratch of the largest group (the groups run one after another)
+  DevBuf roots;  // 32 bytes per batched instance
+  hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t uses = 0;
   ~CommitPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    for (CommitGroup& g : groups)
+      if (g.done) (void)hipEventDestroy(g.done);
   }
 };
+
+// Side streams of the calling thread (default priority), one per concurrent group of
+// a batched commit; they fork from and join back into the thread's stream.
+struct SideStreams {
+  std::vector<hipStream_t> s;
+  ~SideStreams() { release(); }
+  void release() {
+    for (hipStream_t x : s) (void)hipStreamDestroy(x);
+    s.clear();
+  }
+  hipStream_t get(size_t i) {
+    while (s.size() <= i) {
+      hipStream_t x = nullptr;
+      WHIR_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+      s.push_back(x);
+    }
+    return s[i];
+  }
+};
+thread_local SideStreams t_side;
+constexpr size_t MAX_SIDE_STREAMS = 8;
 
 // Plans of the calling thread's stream, most recently used first (a few layouts).
 thread_local std::vector<std::unique_ptr<CommitPlan>> t_plans;
@@ -582,7 +608,10 @@ constexpr size_t MAX_PLANS = 4;
 
 // Frees the calling thread's plans (their buffers go back to the thread's stream, their
 // graphs are destroyed); called before the thread's stream is released.
-void release_commit_plans() { t_plans.clear(); }
+void release_commit_plans() {
+  t_plans.clear();
+  t_side.release();
+}
 
 namespace {
 
@@ -593,8 +622,14 @@ bool batchable(const WhirCfg& c) {
          (N & (N - 1)) == 0 && N % L == 0;
 }
 
+// The groups' launches, each group on a side stream forked from s and joined back
+// (in a capture these become parallel branches of the graph).
 void run_commit_plan(CommitPlan& P, hipStream_t s) {
-  for (CommitGroup& g : P.groups) {
+  WHIR_HIP(hipEventRecord(P.fork, s));
+  for (size_t gi = 0; gi < P.groups.size(); gi++) {
+    CommitGroup& g = P.groups[gi];
+    const hipStream_t gs = t_side.get(gi % MAX_SIDE_STREAMS);
+    WHIR_HIP(hipStreamWaitEvent(gs, P.fork, 0));
     const CommitShape& sh = g.shape;
     const size_t cnt = g.members.size();
     const uint64_t* const* msg_tab = (const uint64_t* const*)g.tabs.ptr;
@@ -604,10 +639,12 @@ void run_commit_plan(CommitPlan& P, hipStream_t s) {
     a.msg_tab = msg_tab;
     a.out_tab = cw_tab;
     a.mpi = sh.M;
-    rs_encode_cols(1, a, P.scratch.as<uint64_t>(), s);
+    rs_encode_cols(1, a, g.scratch.as<uint64_t>(), gs);
     commit_cols_many_dev(sh.engine, 1, (const uint64_t* const*)cw_tab, nodes_tab, cnt, sh.N, sh.M,
-                         P.roots.as<uint8_t>() + 32 * g.root0, s);
+                         P.roots.as<uint8_t>() + 32 * g.root0, gs);
+    WHIR_HIP(hipEventRecord(g.done, gs));
   }
+  for (CommitGroup& g : P.groups) WHIR_HIP(hipStreamWaitEvent(s, g.done, 0));
 }
 
 CommitPlan& commit_plan(const std::vector<CommitShape>& layout, hipStream_t s) {
@@ -627,16 +664,17 @@ CommitPlan& commit_plan(const std::vector<CommitShape>& layout, hipStream_t s) {
     }
     it->members.push_back(k);
   }
-  size_t slot = 0, scratch = 0;
+  size_t slot = 0;
   for (CommitGroup& g : P->groups) {
     const size_t cnt = g.members.size();
     g.root0 = slot;
     slot += cnt;
     g.tabs = DevBuf(3 * cnt * 8, s);
-    scratch = std::max(scratch, rs_scratch_words(1, g.shape.M * cnt, g.shape.N, g.shape.L));
+    g.scratch = DevBuf(rs_scratch_words(1, g.shape.M * cnt, g.shape.N, g.shape.L) * 8, s);
+    WHIR_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
   }
   P->roots = DevBuf(32 * layout.size(), s);
-  P->scratch = DevBuf(scratch * 8, s);
+  WHIR_HIP(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
   if (t_plans.size() >= MAX_PLANS) t_plans.pop_back();
   t_plans.insert(t_plans.begin(), std::move(P));
   return *t_plans[0];
