@@ -53,6 +53,10 @@ int whir_set_device(int device);
 
 int whir_dev_alloc(size_t bytes, void **ptr);
 int whir_dev_free(void *ptr);
+/* Pinned (page-locked) host memory, for host-memory inputs that overlap their upload
+ * with device work (whir_commit_host). */
+int whir_host_alloc(size_t bytes, void **ptr);
+int whir_host_free(void *ptr);
 int whir_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int whir_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int whir_memcpy_d2d(void *dst, const void *src, size_t bytes);
@@ -293,6 +297,15 @@ void whir_transcript_free(whir_transcript *t);
 /* whir::Config::commit of a device-resident base-field vector of `size` words.
  * The witness (codeword and Merkle tree) stays on the device. */
 int whir_commit_dev(const whir_config *cfg, whir_transcript *t, const uint64_t *vector, whir_witness **out);
+/* whir::Config::commit of a vector in host memory (irs_commit::Config::commit takes
+ * host slices, src/protocols/irs_commit.rs:292-350): the coefficients are uploaded into
+ * dev_vector (vector size words, allocated by the caller and kept for whir_prove_dev)
+ * a chunk of messages at a time on a second stream, each chunk's Reed-Solomon encode
+ * starting as soon as its copy has landed.  host_vector in pinned memory
+ * (whir_host_alloc) overlaps best.  Same witness and transcript as uploading the
+ * vector and calling whir_commit_dev. */
+int whir_commit_host(const whir_config *cfg, whir_transcript *t, const uint64_t *host_vector, uint64_t *dev_vector,
+                     whir_witness **out);
 /* whir::Config::commit of `count` independent instances (each its own configuration
  * with batch_size 1, transcript and device vector), with results identical to `count`
  * calls of whir_commit_dev.  No counterpart in the reference (which commits one

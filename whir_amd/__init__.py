@@ -58,6 +58,9 @@ SIGNATURES = {
     "whir_set_device": [ci],
     "whir_dev_alloc": [sz, ctypes.POINTER(vp)],
     "whir_dev_free": [vp],
+    "whir_host_alloc": [sz, ctypes.POINTER(vp)],
+    "whir_host_free": [vp],
+    "whir_commit_host": [vp, vp, vp, vp, ctypes.POINTER(vp)],
     "whir_memcpy_h2d": [vp, vp, sz],
     "whir_memcpy_d2h": [vp, vp, sz],
     "whir_memcpy_d2d": [vp, vp, sz],
@@ -469,6 +472,29 @@ class DeviceBuffer:
     def close(self):
         if self.ptr:
             lib().whir_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A numpy uint64 array in pinned host memory (whir_host_alloc)."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        p = vp()
+        _call("whir_host_alloc", max(self.n, 1) * 8, ctypes.byref(p))
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array(ctypes.cast(self.ptr, u64p), shape=(max(self.n, 1),))[: self.n]
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().whir_host_free(self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -122,6 +122,16 @@ int whir_dev_free(void* ptr) {
   WHIR_HIP(hipFree(ptr));
   API_END
 }
+int whir_host_alloc(size_t bytes, void** ptr) {
+  API_BEGIN
+  WHIR_HIP(hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+  API_END
+}
+int whir_host_free(void* ptr) {
+  API_BEGIN
+  WHIR_HIP(hipHostFree(ptr));
+  API_END
+}
 int whir_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   API_BEGIN
   hipStream_t s = thread_stream();

@@ -46,6 +46,10 @@ using gl::Ext;
 #endif
 static constexpr int P1_THREADS = 256;
 static constexpr int NTT_THREADS = 512;
+// Pass 2 of Fp3 runs 256-thread blocks (1.46 -> 1.27-1.31 ms for 2^26 outputs on
+// MI355X: Fp3 rows fill fewer waves per tile); Fp and Fp2 keep 512 (256 measured 2-14% slower).
+template <int D>
+constexpr int pass2_threads() { return D == 3 ? 256 : NTT_THREADS; }
 static constexpr int MAX_PASS_LOG = 11;   // largest DFT staged in one pass (a 2^11 Fp3
                                           // column is 48 KiB, +16 KiB of twiddles)
 
@@ -382,20 +386,22 @@ __device__ __forceinline__ void lds_dft(Ext<D>* s, const uint64_t* tw, int logS,
   if (R16) {
     while (logSub >= 4) {
       dif_level<D, PAD, 16, 4>(s, tw, logS, logSub, cols);
-      __syncthreads();
       logSub -= 4;
+      __syncthreads();
     }
   }
   while (logSub >= 3) {
     dif_level<D, PAD, 8, 3>(s, tw, logS, logSub, cols);
-    __syncthreads();
     logSub -= 3;
+    __syncthreads();
   }
   if (logSub == 2) {
     dif_step<D, PAD, 4, 2, false>(s, tw, logS, logSub, cols);
+    logSub -= 2;
     __syncthreads();
   } else if (logSub == 1) {
     dif_step<D, PAD, 2, 1, false>(s, tw, logS, logSub, cols);
+    logSub -= 1;
     __syncthreads();
   }
 }
@@ -759,7 +765,7 @@ static void pass2_t(RsArgs a, uint64_t* scratch, int logC1, int tw, hipStream_t 
   dim3 grid(xcd_grid((uint64_t)(C2 >> logb) * a.Q * a.M));
   allow_lds(k_ntt_pass2<D, PAD, R16>, smem);
   ProfScope prof(kernel_name("k_ntt_pass2", D), st, 16.0 * D * (double)a.M * a.N);
-  k_ntt_pass2<D, PAD, R16><<<grid, NTT_THREADS, smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
+  k_ntt_pass2<D, PAD, R16><<<grid, pass2_threads<D>(), smem, st>>>(a, flat_twiddles(logC1), scratch, logC1, logb);
   WHIR_HIP(hipGetLastError());
 }
 

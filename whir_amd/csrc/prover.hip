@@ -150,9 +150,47 @@ uint64_t coset_size(uint64_t masked_len, uint64_t N) {
 // irs_commit::Config::commit, src/protocols/irs_commit.rs:292-350 (no masks):
 // vecs[0..nvec) each hold vector_size elements of Ds words on the device; vector
 // b fills messages (columns) [b*depth, (b+1)*depth) of the committed matrix.
+// The host side of a commit whose vector arrives from host memory: the coefficients go
+// up a chunk of messages at a time on a copy stream, and each chunk's encode starts as
+// soon as its copy has landed (the encode of message j needs message j only), so the
+// upload overlaps the encodes of the earlier chunks.
+struct HostUpload {
+  const uint64_t* host;  // the vector in host memory (pinned memory overlaps best)
+  uint64_t* dev;         // its device copy, filled by the upload
+};
+
+// A second stream per thread for host->device copies (default priority).
+thread_local struct CopyStream {
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> ev;
+  ~CopyStream() { release(); }
+  void release() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    if (s) (void)hipStreamDestroy(s);
+    s = nullptr;
+  }
+  hipStream_t get() {
+    if (!s) WHIR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }
+  hipEvent_t event(size_t i) {
+    while (ev.size() <= i) {
+      hipEvent_t e = nullptr;
+      WHIR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ev.push_back(e);
+    }
+    return ev[i];
+  }
+} t_copy;
+
+// Messages per uploaded chunk of a host-memory commit (4 chunks of 4 messages at the
+// usual interleaving depth 16).
+constexpr size_t UPLOAD_CHUNK_MESSAGES = 4;
+
 template <int D>
 void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* const* vecs, size_t nvec, int Ds,
-                DevWitness& w, hipStream_t s) {
+                DevWitness& w, hipStream_t s, const HostUpload* up = nullptr) {
   WHIR_REQUIRE(nvec == c.num_vectors, "irs commit: the configuration commits to a different number of vectors");
   const size_t depth = c.interleaving_depth, M = c.num_messages(), L = c.message_length(), N = c.codeword_length;
   uint64_t C = coset_size(L, N);
@@ -161,7 +199,23 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   w.M = M;
   w.num_vectors = nvec;
   w.cw = DevBuf(M * N * Ds * 8, s);
-  {
+  if (up) {
+    WHIR_REQUIRE(nvec == 1 && Ds == 1, "irs commit: host upload of one base-field vector only");
+    const size_t G = std::min(UPLOAD_CHUNK_MESSAGES, depth);
+    DevBuf scratch(rs_scratch_words(1, G, N, C) * 8, s);
+    const hipStream_t cs = t_copy.get();
+    // the copy stream must not overwrite the device vector before earlier work on s is done
+    WHIR_HIP(hipEventRecord(t_copy.event(0), s));
+    WHIR_HIP(hipStreamWaitEvent(cs, t_copy.event(0), 0));
+    for (size_t j0 = 0, k = 1; j0 < depth; j0 += G, k++) {
+      const size_t g = std::min(G, depth - j0);
+      WHIR_HIP(hipMemcpyAsync(up->dev + j0 * L, up->host + j0 * L, g * L * 8, hipMemcpyHostToDevice, cs));
+      WHIR_HIP(hipEventRecord(t_copy.event(k), cs));
+      WHIR_HIP(hipStreamWaitEvent(s, t_copy.event(k), 0));
+      RsArgs a{up->dev + j0 * L, L, L, nullptr, 0, w.cw.as<uint64_t>() + j0 * N, N, N / C, log2_exact(C), g};
+      rs_encode_cols(1, a, scratch.as<uint64_t>(), s);
+    }
+  } else {
     DevBuf scratch(rs_scratch_words(Ds, depth, N, C) * 8, s);
     for (size_t b = 0; b < nvec; b++) {
       RsArgs a{vecs[b], L, L, nullptr, 0, w.cw.as<uint64_t>() + b * depth * N * Ds, N, N / C, log2_exact(C), depth};
@@ -611,6 +665,7 @@ constexpr size_t MAX_PLANS = 4;
 void release_commit_plans() {
   t_plans.clear();
   t_side.release();
+  t_copy.release();
 }
 
 namespace {
@@ -1133,6 +1188,24 @@ int whir_commit_batch_dev(const whir_config* c, whir_transcript* t, const uint64
 
 int whir_commit_dev(const whir_config* c, whir_transcript* t, const uint64_t* vector, whir_witness** out) {
   return whir_commit_batch_dev(c, t, &vector, 1, out);
+}
+
+int whir_commit_host(const whir_config* c, whir_transcript* t, const uint64_t* host_vector, uint64_t* dev_vector,
+                     whir_witness** out) {
+  API_BEGIN
+  WHIR_REQUIRE(c && t && host_vector && dev_vector && out, "commit_host: null argument");
+  WHIR_REQUIRE(c->cfg.initial.num_vectors == 1, "commit_host: configurations of batch_size 1 only");
+  hipStream_t s = thread_stream();
+  auto w = std::make_unique<whir_witness>();
+  HostUpload up{host_vector, dev_vector};
+  const uint64_t* v = dev_vector;
+  switch (c->cfg.D) {
+    case 1: irs_commit<1>(*t->t, c->cfg.initial, c->cfg.engine, &v, 1, 1, w->w, s, &up); break;
+    case 2: irs_commit<2>(*t->t, c->cfg.initial, c->cfg.engine, &v, 1, 1, w->w, s, &up); break;
+    case 3: irs_commit<3>(*t->t, c->cfg.initial, c->cfg.engine, &v, 1, 1, w->w, s, &up); break;
+  }
+  *out = w.release();
+  API_END
 }
 
 int whir_commit_many_dev(const whir_config* const* cfgs, whir_transcript* const* ts, const uint64_t* const* vectors,

@@ -200,6 +200,19 @@ def commit(cfg: Config, t: Transcript, d_vector: DeviceBuffer) -> Witness:
     return Witness(h.value)
 
 
+def commit_host(cfg: Config, t: Transcript, host_vector, d_vector: DeviceBuffer) -> Witness:
+    """whir::Config::commit of a vector in host memory (a numpy uint64 array or a
+    PinnedArray): uploaded into d_vector a chunk of messages at a time, each chunk's
+    encode overlapping the later chunks' copies."""
+    from . import PinnedArray
+    arr = None if isinstance(host_vector, PinnedArray) else np.ascontiguousarray(host_vector, dtype=np.uint64)
+    ptr = host_vector.ptr if arr is None else arr.ctypes.data
+    h = vp()
+    _call("whir_commit_host", cfg.h, t.h, ptr, d_vector.ptr, ctypes.byref(h))  # returns after the copies
+    del arr
+    return Witness(h.value)
+
+
 def commit_many(cfgs, transcripts, d_vectors):
     """whir::Config::commit of many independent instances in shared launches (a HIP
     graph per batch layout); the same witnesses and transcripts as commit() on each."""
