@@ -34,29 +34,45 @@ __device__ __forceinline__ void st(uint64_t* p, size_t i, const Ext<D>& v) {
 template <int D>
 __device__ __forceinline__ Ext<D> ld_host(const uint64_t* const* unused, const Ext<D>& v) { return v; }
 
-// Sum NV extension values over the block; thread 0 writes them to out.
+// Sum NV extension values over the block; thread 0 writes them to out.  Each wave sums
+// its lanes by cross-lane exchanges (no barrier), its first lane puts the wave's sum in
+// LDS, and after one barrier thread 0 adds the waves' sums.
 template <int D, int NV>
 __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out) {
-  __shared__ uint64_t red[RT * NV * D];
+  __shared__ uint64_t red[RT / 64][NV * D];
 #pragma unroll
-  for (int k = 0; k < NV; k++)
+  for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
-    for (int d = 0; d < D; d++) red[(threadIdx.x * NV + k) * D + d] = v[k].c[d];
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
+    for (int k = 0; k < NV; k++) {
+      Ext<D> o;
 #pragma unroll
-      for (int k = 0; k < NV; k++)
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-          size_t a = (threadIdx.x * NV + k) * D + d, b = ((threadIdx.x + s) * NV + k) * D + d;
-          red[a] = gl::add(red[a], red[b]);
-        }
+      for (int d = 0; d < D; d++) o.c[d] = __shfl_xor(v[k].c[d], off);
+      v[k] = gl::eadd<D>(v[k], o);
     }
-    __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+      for (int d = 0; d < D; d++) red[wave][k * D + d] = v[k].c[d];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int waves = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      Ext<D> acc;
+#pragma unroll
+      for (int d = 0; d < D; d++) acc.c[d] = red[0][k * D + d];
+      for (int w = 1; w < waves; w++) {
+        Ext<D> x;
+#pragma unroll
+        for (int d = 0; d < D; d++) x.c[d] = red[w][k * D + d];
+        acc = gl::eadd<D>(acc, x);
+      }
+#pragma unroll
+      for (int d = 0; d < D; d++) out[k * D + d] = acc.c[d];
+    }
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < NV * D; k++) out[k] = red[k];
 }
 
 // Final reduction of `nparts` block partials of NV*D words each into `out`
