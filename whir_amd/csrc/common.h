@@ -52,9 +52,17 @@ void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Pinned host words (RESULT_MAILBOX_WORDS per stream) that kernels write small
 // results into directly, saving a device-to-host copy per result; read them after
-// synchronizing the stream.
+// mailbox_wait (or a stream synchronization).  The words are fine-grained (coherent)
+// host memory, so device stores to them are not held in the GPU's L2.
 constexpr size_t RESULT_MAILBOX_WORDS = 65536;
 uint64_t* result_mailbox(hipStream_t s);
+// Waits until everything queued on s so far has finished, for a host that is about to
+// read the mailbox: a one-thread kernel queued behind that work stores a new sequence
+// number to a coherent flag word with a system-scope release, and the host spins on
+// the flag (about 6 us from kernel end to host, against about 12 us for
+// hipStreamSynchronize on MI355X).  A stream error ends the wait with that error.
+// Not for use while s is being captured.
+void mailbox_wait(hipStream_t s);
 
 // Device buffers recycled per stream: a buffer released on stream s goes to s's
 // free list and is handed out again only to allocations on s, so stream order

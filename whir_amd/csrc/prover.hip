@@ -227,7 +227,7 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   // The root lands in the stream's pinned mailbox from the last tree launch.
   uint64_t* mbox = result_mailbox(s);
   commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
-  WHIR_HIP(hipStreamSynchronize(s));
+  mailbox_wait(s);
   uint8_t root[32];
   memcpy(root, mbox, 32);
   t.prover_hash(root);
@@ -243,7 +243,7 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
     for (size_t b = 0; b < nvec; b++)
       univariate_eval_many_dev(Ds, D, vecs[b], c.vector_size, w.ood_points.data(), nood, mbox + b * pad * D, s,
                                true);
-    WHIR_HIP(hipStreamSynchronize(s));
+    mailbox_wait(s);
     w.ood_values.resize(nood * nvec * D);
     for (size_t i = 0; i < nood; i++)
       for (size_t b = 0; b < nvec; b++) {
@@ -439,7 +439,7 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
                                        const std::function<void()>& after_first) {
   bool first = true;
   auto synced = [&] {
-    WHIR_HIP(hipStreamSynchronize(s));
+    mailbox_wait(s);
     if (first && after_first) after_first();
     first = false;
   };
@@ -536,7 +536,7 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     } else {
       sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, n, mbox, s);
     }
-    WHIR_HIP(hipStreamSynchronize(s));
+    mailbox_wait(s);
     if (first && after_first) after_first();
     first = false;
     uint64_t h[2 * D];
@@ -558,7 +558,7 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
     lift_dev(D, a_src, n, a, s);
   }
   if (first && after_first) {
-    WHIR_HIP(hipStreamSynchronize(s));
+    mailbox_wait(s);
     after_first();
   }
   return chals;
@@ -887,7 +887,7 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     WHIR_REQUIRE(V * pad * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain cross terms");
     for (size_t j = 0; j < V; j++)
       univariate_eval_many_dev(1, D, vecs[j], n, opts.data(), nood, mbox + j * pad * D, s, true);
-    WHIR_HIP(hipStreamSynchronize(s));
+    mailbox_wait(s);
     cross.resize(V * nood * D);
     for (size_t j = 0; j < V; j++) std::copy(mbox + j * pad * D, mbox + (j * pad + nood) * D, &cross[j * nood * D]);
   }

@@ -380,18 +380,56 @@ std::string profile_json() {
 }
 
 namespace {
+// The flag word of mailbox_wait sits in its own 64-byte line after the mailbox.
+constexpr size_t MAILBOX_FLAG_WORD = RESULT_MAILBOX_WORDS + 8;
 struct StreamSlots {
   uint64_t* mailbox = nullptr;
+  uint64_t seq = 0;  // last sequence number mailbox_wait queued on the stream
 };
 std::mutex g_slots_mu;
 std::map<hipStream_t, StreamSlots> g_slots;
-}  // namespace
 
-uint64_t* result_mailbox(hipStream_t s) {
+StreamSlots& stream_slots(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_slots_mu);
   StreamSlots& sl = g_slots[s];
-  if (!sl.mailbox) WHIR_HIP(hipHostMalloc((void**)&sl.mailbox, RESULT_MAILBOX_WORDS * 8, hipHostMallocDefault));
-  return sl.mailbox;
+  if (!sl.mailbox) {
+    WHIR_HIP(hipHostMalloc((void**)&sl.mailbox, (MAILBOX_FLAG_WORD + 8) * 8, hipHostMallocCoherent));
+    sl.mailbox[MAILBOX_FLAG_WORD] = 0;
+  }
+  return sl;  // std::map references stay valid until the entry is erased
+}
+
+__global__ void k_mailbox_signal(uint64_t* flag, uint64_t seq) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+}  // namespace
+
+uint64_t* result_mailbox(hipStream_t s) { return stream_slots(s).mailbox; }
+
+void mailbox_wait(hipStream_t s) {
+  WHIR_REQUIRE(!capture_active(), "mailbox_wait during a stream capture");
+  StreamSlots& sl = stream_slots(s);
+  const uint64_t want = ++sl.seq;
+  uint64_t* flag = sl.mailbox + MAILBOX_FLAG_WORD;
+  k_mailbox_signal<<<1, 64, 0, s>>>(flag, want);
+  WHIR_HIP(hipGetLastError());
+  for (uint32_t n = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != want; n++) {
+    if ((n & 4095) == 0) {
+      // Every few thousand polls ask the runtime: an error ends the wait with that
+      // error; an idle stream whose flag has not shown yet falls back to a plain
+      // synchronization.
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipSuccess) {
+        WHIR_HIP(hipStreamSynchronize(s));
+        break;
+      }
+      if (e != hipErrorNotReady) WHIR_HIP(e);
+    }
+    __builtin_ia32_pause();
+  }
 }
 
 static void release_stream_slots(hipStream_t s) {
