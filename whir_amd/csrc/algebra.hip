@@ -34,11 +34,12 @@ __device__ __forceinline__ void st(uint64_t* p, size_t i, const Ext<D>& v) {
 template <int D>
 __device__ __forceinline__ Ext<D> ld_host(const uint64_t* const* unused, const Ext<D>& v) { return v; }
 
-// Sum NV extension values over the block; thread 0 writes them to out.  Each wave sums
-// its lanes by cross-lane exchanges (no barrier), its first lane puts the wave's sum in
-// LDS, and after one barrier thread 0 adds the waves' sums.
+// Sum NV extension values over the block; thread 0 writes them to out (and then raises
+// the mailbox flag when sig asks it to).  Each wave sums its lanes by cross-lane
+// exchanges (no barrier), its first lane puts the wave's sum in LDS, and after one
+// barrier thread 0 adds the waves' sums.
 template <int D, int NV>
-__device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out) {
+__device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out, MailboxSig sig = {}) {
   __shared__ uint64_t red[RT / 64][NV * D];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1)
@@ -72,6 +73,7 @@ __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out) {
 #pragma unroll
       for (int d = 0; d < D; d++) out[k * D + d] = acc.c[d];
     }
+    mailbox_raise(sig);
   }
 }
 
@@ -81,19 +83,21 @@ __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out) {
 // release that pattern needs writes back the XCD's L2, and the producing kernels
 // have just filled it with their outputs (measured 1.5x slower on MI355X).
 template <int D, int NV>
-__global__ __launch_bounds__(RT) void k_reduce_parts(const uint64_t* parts, size_t nparts, uint64_t* out) {
+__global__ __launch_bounds__(RT) void k_reduce_parts(const uint64_t* parts, size_t nparts, uint64_t* out,
+                                                     MailboxSig sig) {
   Ext<D> v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = gl::ext_zero<D>();
   for (size_t p = threadIdx.x; p < nparts; p += blockDim.x)
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = gl::eadd<D>(v[k], ld<D>(parts + p * NV * D, k));
-  block_reduce_store<D, NV>(v, out);
+  block_reduce_store<D, NV>(v, out, sig);
 }
 
+// last: this launch ends the work a mailbox wait may be armed for.
 template <int D, int NV>
-static void finish_reduce(const uint64_t* parts, unsigned nparts, uint64_t* out, hipStream_t s) {
-  k_reduce_parts<D, NV><<<1, RT, 0, s>>>(parts, nparts, out);
+static void finish_reduce(const uint64_t* parts, unsigned nparts, uint64_t* out, hipStream_t s, bool last = true) {
+  k_reduce_parts<D, NV><<<1, RT, 0, s>>>(parts, nparts, out, last ? mailbox_take(out) : MailboxSig{});
   WHIR_HIP(hipGetLastError());
 }
 
@@ -135,7 +139,7 @@ __device__ __forceinline__ Ext<D> mulx(const uint64_t* a, size_t i, const Ext<D>
 
 template <int Ds, int D>
 __global__ __launch_bounds__(RT) void k_sc_poly(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
-                                                size_t half, size_t nhi, uint64_t* parts) {
+                                                size_t half, size_t nhi, uint64_t* parts, MailboxSig sig) {
   Ext<D> acc[2] = {gl::ext_zero<D>(), gl::ext_zero<D>()};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x) {
     const Ext<D> b0 = ld<D>(b, i);
@@ -152,21 +156,22 @@ __global__ __launch_bounds__(RT) void k_sc_poly(const uint64_t* __restrict__ a, 
       acc[1] = gl::eadd<D>(acc[1], p0);  // implicit zero padding: (0 - a)(0 - b) = a b
     }
   }
-  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D);
+  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D, sig);
 }
 
 template <int Ds, int D>
-__global__ void k_sc_single(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out) {
+__global__ void k_sc_single(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out, MailboxSig sig) {
   if (threadIdx.x != 0) return;
   Ext<D> c0 = n ? mulx<Ds, D>(a, 0, ld<D>(b, 0)) : gl::ext_zero<D>();
   st<D>(out, 0, c0);
   st<D>(out, 1, gl::ext_zero<D>());
+  mailbox_raise(sig);
 }
 
 template <int Ds, int D>
 static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* out, hipStream_t s) {
   if (n <= 1) {
-    k_sc_single<Ds, D><<<1, 64, 0, s>>>(a, b, n, out);
+    k_sc_single<Ds, D><<<1, 64, 0, s>>>(a, b, n, out, mailbox_take(out));
     WHIR_HIP(hipGetLastError());
     return;
   }
@@ -176,7 +181,8 @@ static void sc_poly(const uint64_t* a, const uint64_t* b, size_t n, uint64_t* ou
   DevBuf parts(g == 1 ? 0 : (size_t)g * 2 * D * 8, s);
   {
     ProfScope prof(kernel_name("k_sc_poly", Ds, D), s, 8.0 * (Ds + D) * (double)n);
-    k_sc_poly<Ds, D><<<g, RT, 0, s>>>(a, b, half, nhi, g == 1 ? out : parts.as<uint64_t>());
+    k_sc_poly<Ds, D><<<g, RT, 0, s>>>(a, b, half, nhi, g == 1 ? out : parts.as<uint64_t>(),
+                                      g == 1 ? mailbox_take(out) : MailboxSig{});
     WHIR_HIP(hipGetLastError());
   }
   if (g > 1) finish_reduce<D, 2>(parts.as<uint64_t>(), g, out, s);
@@ -318,7 +324,8 @@ size_t fold_both_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint6
 // i, h/2 + i, h + i, h + h/2 + i, so a_out may alias a_in when Ds == D.
 template <int Ds, int D>
 __global__ __launch_bounds__(RT) void k_fold_poly(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t h,
-                                                  size_t nhi, Ext<D> w, Ext<D> om, uint64_t* parts) {
+                                                  size_t nhi, Ext<D> w, Ext<D> om, uint64_t* parts,
+                                                  MailboxSig sig) {
   Ext<D> acc[2] = {gl::ext_zero<D>(), gl::ext_zero<D>()};
   const size_t q = h >> 1;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(RT) void k_fold_poly(const uint64_t* a_in, uint64_t
     acc[0] = gl::eadd<D>(acc[0], gl::emul<D>(a0, b0));
     acc[1] = gl::eadd<D>(acc[1], gl::emul<D>(gl::esub<D>(a1, a0), gl::esub<D>(b1, b0)));
   }
-  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D);
+  block_reduce_store<D, 2>(acc, parts + (size_t)blockIdx.x * 2 * D, sig);
 }
 
 // Block partials of one fold + polynomial pass: none when one block does it all.
@@ -359,7 +366,8 @@ static size_t fold_poly_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, si
   uint64_t* pp = parts_buf ? parts_buf : parts.as<uint64_t>();
   {
     ProfScope prof(kernel_name("k_fold_poly", Ds, D), s, 8.0 * ((Ds + D) * (double)n + 2.0 * D * h));
-    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, g == 1 ? out : pp);
+    k_fold_poly<Ds, D><<<g, RT, 0, s>>>(a_in, a_out, b, h, nhi, ww, om, g == 1 ? out : pp,
+                                        g == 1 ? mailbox_take(out) : MailboxSig{});
     WHIR_HIP(hipGetLastError());
   }
   if (g > 1) finish_reduce<D, 2>(pp, g, out, s);
@@ -442,7 +450,7 @@ __device__ __forceinline__ Ext<D> fold_twice(const uint64_t* v, size_t t, size_t
 
 template <int Ds, int D, bool FOLD, int NS>
 __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
-                                                Ext<D> r1, Ext<D> r2, uint64_t* parts) {
+                                                Ext<D> r1, Ext<D> r2, uint64_t* parts, MailboxSig sig) {
   constexpr int NA = NS > 0 ? NS : 1;
   Ext<D> acc[NA];
 #pragma unroll
@@ -483,7 +491,7 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
       acc[6] = gl::eadd<D>(acc[6], gl::eadd<D>(gl::escale<D>(vy, ux), gl::escale<D>(uy, vx)));
       acc[7] = gl::eadd<D>(acc[7], gl::escale<D>(vy, vx));
     }
-    block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D);
+    block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D, sig);
     return;
   }
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
@@ -524,7 +532,7 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
       }
     }
   }
-  if constexpr (NS > 0) block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D);
+  if constexpr (NS > 0) block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D, sig);
 }
 
 template <int Ds, int D, bool FOLD, int NS>
@@ -537,8 +545,9 @@ static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t
   {
     ProfScope prof(kernel_name(FOLD ? (NS ? "k_sc_pair_fold" : "k_fold4") : "k_sc_pair", Ds, D), s,
                    FOLD ? 8.0 * ((Ds + D) * 4.0 * L + 2.0 * D * L) : 8.0 * (Ds + D) * (double)L);
-    k_sc_pair<Ds, D, FOLD, NS><<<g, RT, 0, s>>>(a_in, a_out, b, L, w1, w2,
-                                                NS > 0 && g > 1 ? parts.as<uint64_t>() : out);
+    const bool direct = NS > 0 && g == 1;  // one block stores the sums itself
+    k_sc_pair<Ds, D, FOLD, NS><<<g, RT, 0, s>>>(a_in, a_out, b, L, w1, w2, direct ? out : parts.as<uint64_t>(),
+                                                direct ? mailbox_take(out) : MailboxSig{});
     WHIR_HIP(hipGetLastError());
   }
   if constexpr (NS > 0) {
@@ -938,7 +947,7 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
       k_uni<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, buf.as<uint64_t>(), L, (int)mj, parts.as<uint64_t>());
       WHIR_HIP(hipGetLastError());
     }
-    finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s);
+    finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s, j0 + UNI_MP >= m);
     if (dst != o) WHIR_HIP(hipMemcpyAsync(o, dst, mj * D * 8, hipMemcpyDeviceToDevice, s));
   }
 }

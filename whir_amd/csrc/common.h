@@ -57,12 +57,38 @@ void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
 constexpr size_t RESULT_MAILBOX_WORDS = 65536;
 uint64_t* result_mailbox(hipStream_t s);
 // Waits until everything queued on s so far has finished, for a host that is about to
-// read the mailbox: a one-thread kernel queued behind that work stores a new sequence
-// number to a coherent flag word with a system-scope release, and the host spins on
-// the flag (about 6 us from kernel end to host, against about 12 us for
-// hipStreamSynchronize on MI355X).  A stream error ends the wait with that error.
-// Not for use while s is being captured.
+// read the mailbox: a new sequence number is stored to a coherent flag word after the
+// mailbox with a system-scope release, and the host spins on the flag (about 6 us from
+// kernel end to host, against about 12 us for hipStreamSynchronize on MI355X).  The
+// store comes from the kernel that wrote the awaited result when the wait was armed
+// (mailbox_arm) and that kernel took the request (mailbox_take), else from a
+// one-thread kernel queued behind the stream's work.  A stream error ends the wait
+// with that error.  Not for use while s is being captured.
 void mailbox_wait(hipStream_t s);
+
+// A request that a kernel storing a result into the mailbox also raise the flag.
+struct MailboxSig {
+  uint64_t* flag = nullptr;
+  uint64_t seq = 0;
+};
+// Arms the calling thread's next mailbox_wait on s: the launch that stores the awaited
+// result may take the request and raise the flag itself, which saves the separate
+// signal kernel.  Call it right before the call whose LAST launch stores that result,
+// so that nothing the host reads is still being written when the flag rises.
+void mailbox_arm(hipStream_t s);
+// The armed request when `out` (the destination of the launch about to be made) lies
+// in the armed stream's mailbox, which disarms it; otherwise an empty request.  Only a
+// launch that is the last of the awaited work may take it.
+MailboxSig mailbox_take(const void* out);
+#if defined(__HIP__)
+// Device side, by the one thread that stored the result (after its stores).
+__device__ inline void mailbox_raise(const MailboxSig& sig) {
+  if (sig.flag) {
+    __threadfence_system();
+    __hip_atomic_store(sig.flag, sig.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+#endif
 
 // Device buffers recycled per stream: a buffer released on stream s goes to s's
 // free list and is handed out again only to allocations on s, so stream order

@@ -287,7 +287,8 @@ static int merkle_gmax_rt(int) { return 3; }
 
 template <int ENG>
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restrict__ nodes, size_t W, size_t w,
-                                                               int L, int G1, uint32_t* root_out, BatchTab tab) {
+                                                               int L, int G1, uint32_t* root_out, BatchTab tab,
+                                                               MailboxSig sig) {
   constexpr int NH = 1 << merkle_gmax<ENG>();
   __shared__ uint32_t buf[2][MERKLE_THREADS][8];
   size_t blk = blockIdx.x;
@@ -343,7 +344,11 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restri
     left -= G;
   }
   // The launch that makes the root also hands it to the caller (e.g. mapped host memory).
-  if (root_out && threadIdx.x < 8) root_out[threadIdx.x] = buf[cur][0][threadIdx.x];
+  if (root_out && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) root_out[i] = buf[cur][0][i];
+    mailbox_raise(sig);
+  }
 }
 
 // Merkle levels above a level of width w (a power of two) already in place.  The
@@ -369,8 +374,11 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)) * count);
     tab.per = (uint32_t)blocks;
     uint32_t* ro = blocks == 1 ? (tab.nodes ? (uint32_t*)tab.roots : (uint32_t*)root_out) : nullptr;
+    // The launch that makes a single tree's root is the last of the commit.
+    const MailboxSig sig = blocks == 1 && !tab.nodes && ro ? mailbox_take(ro) : MailboxSig{};
     engine_switch(engine, [&](auto E) {
-      k_merkle_sub<decltype(E)::value><<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro, tab);
+      k_merkle_sub<decltype(E)::value><<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro, tab,
+                                                                                       sig);
     });
     WHIR_HIP(hipGetLastError());
     w >>= L;

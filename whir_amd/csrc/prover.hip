@@ -226,6 +226,7 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   w.nodes = DevBuf(32 * nn, s);
   // The root lands in the stream's pinned mailbox from the last tree launch.
   uint64_t* mbox = result_mailbox(s);
+  mailbox_arm(s);
   commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
   mailbox_wait(s);
   uint8_t root[32];
@@ -240,9 +241,11 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
     // (vector b at words [b*pad*D, (b+1)*pad*D)); sent point-major.
     const size_t pad = (nood + 3) / 4 * 4;
     WHIR_REQUIRE(nvec * pad * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain samples");
-    for (size_t b = 0; b < nvec; b++)
+    for (size_t b = 0; b < nvec; b++) {
+      if (b + 1 == nvec) mailbox_arm(s);
       univariate_eval_many_dev(Ds, D, vecs[b], c.vector_size, w.ood_points.data(), nood, mbox + b * pad * D, s,
                                true);
+    }
     mailbox_wait(s);
     w.ood_values.resize(nood * nvec * D);
     for (size_t i = 0; i < nood; i++)
@@ -458,6 +461,7 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     return r;
   };
   while (round < sc.num_rounds) {
+    mailbox_arm(s);  // the pass below ends with its sums in the mailbox
     if (sc.num_rounds - round >= 2 && len >= 16 && pend.size() != 1) {
       // round k's polynomial and round k+1's coefficients, folding the pending pair first
       if (pend.empty()) {
@@ -529,6 +533,7 @@ std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engin
   bool have_prev = false;
   E<D> prev;
   for (size_t round = 0; round < sc.num_rounds; round++) {
+    mailbox_arm(s);
     if (have_prev) {
       n = fold_and_poly_mixed_dev(a_Ds, D, a_src, a, b, n, prev.c, mbox, s);
       a_src = a;
@@ -885,8 +890,10 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
   if (W > 1 && nood) {
     const size_t pad = (nood + 3) / 4 * 4;
     WHIR_REQUIRE(V * pad * D <= RESULT_MAILBOX_WORDS, "too many out-of-domain cross terms");
-    for (size_t j = 0; j < V; j++)
+    for (size_t j = 0; j < V; j++) {
+      if (j + 1 == V) mailbox_arm(s);
       univariate_eval_many_dev(1, D, vecs[j], n, opts.data(), nood, mbox + j * pad * D, s, true);
+    }
     mailbox_wait(s);
     cross.resize(V * nood * D);
     for (size_t j = 0; j < V; j++) std::copy(mbox + j * pad * D, mbox + (j * pad + nood) * D, &cross[j * nood * D]);

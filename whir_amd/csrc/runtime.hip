@@ -2,6 +2,7 @@
 #include "common.h"
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -409,14 +410,67 @@ __global__ void k_mailbox_signal(uint64_t* flag, uint64_t seq) {
 
 uint64_t* result_mailbox(hipStream_t s) { return stream_slots(s).mailbox; }
 
+namespace {
+// The calling thread's armed wait: stream, its mailbox, the sequence number, and
+// whether a launch has taken the request.
+struct ArmedWait {
+  hipStream_t s = nullptr;
+  uint64_t* mbox = nullptr;
+  uint64_t seq = 0;
+  bool armed = false;
+  bool taken = false;
+};
+thread_local ArmedWait t_armed;
+}  // namespace
+
+// WHIR_MAILBOX_SYNC=1: mailbox_wait synchronizes the stream and nothing is armed (the
+// plain wait, kept for comparison).
+static bool mailbox_plain_sync() {
+  static const bool on = [] {
+    const char* e = getenv("WHIR_MAILBOX_SYNC");
+    return e && atoi(e) != 0;
+  }();
+  return on;
+}
+
+void mailbox_arm(hipStream_t s) {
+  if (mailbox_plain_sync()) return;
+  StreamSlots& sl = stream_slots(s);
+  t_armed = ArmedWait{s, sl.mailbox, ++sl.seq, true, false};
+}
+
+MailboxSig mailbox_take(const void* out) {
+  ArmedWait& a = t_armed;
+  const uint64_t* p = (const uint64_t*)out;
+  if (!a.armed || p < a.mbox || p >= a.mbox + RESULT_MAILBOX_WORDS) return {};
+  a.armed = false;
+  a.taken = true;
+  return MailboxSig{a.mbox + MAILBOX_FLAG_WORD, a.seq};
+}
+
 void mailbox_wait(hipStream_t s) {
   WHIR_REQUIRE(!capture_active(), "mailbox_wait during a stream capture");
+  if (mailbox_plain_sync()) {
+    t_armed = ArmedWait{};
+    WHIR_HIP(hipStreamSynchronize(s));
+    return;
+  }
   StreamSlots& sl = stream_slots(s);
-  const uint64_t want = ++sl.seq;
   uint64_t* flag = sl.mailbox + MAILBOX_FLAG_WORD;
-  k_mailbox_signal<<<1, 64, 0, s>>>(flag, want);
-  WHIR_HIP(hipGetLastError());
-  for (uint32_t n = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != want; n++) {
+  uint64_t want;
+  bool signal = true;
+  if (t_armed.s == s && (t_armed.armed || t_armed.taken)) {
+    want = t_armed.seq;
+    signal = !t_armed.taken;
+  } else {
+    want = ++sl.seq;
+  }
+  t_armed = ArmedWait{};
+  if (signal) {
+    k_mailbox_signal<<<1, 64, 0, s>>>(flag, want);
+    WHIR_HIP(hipGetLastError());
+  }
+  for (uint32_t n = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) < want; n++) {
     if ((n & 4095) == 0) {
       // Every few thousand polls ask the runtime: an error ends the wait with that
       // error; an idle stream whose flag has not shown yet falls back to a plain
