@@ -50,6 +50,40 @@ size_t fold_and_poly_mixed_dev(int Ds, int D, const uint64_t* a_in, uint64_t* a_
 void sumcheck_pair_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
                        const uint64_t* r1, const uint64_t* r2, uint64_t* dev_out, hipStream_t st);
 
+// A covector of length n = 2^nv held as nt product terms instead of n values:
+//   y(i) = sum_k hi_k[i >> sh] * lo_k[i & (2^sh - 1)].
+// eq terms factor this way (eq(z, i) is the eq of z's first nv - sh coordinates at the
+// high bits of i times the eq of the rest at the low bits; the term's scalar goes into
+// hi), and so do geometric terms (t x^i = t x^(i_hi 2^sh) * x^(i_lo)).  A sumcheck round
+// folds the top variable, a bit of i_hi while hi has bits left, and the same lo factor
+// multiplies both halves of every pair, so folding y folds the hi tables alone.
+struct CovView {
+  const uint64_t* hi = nullptr;  // nt tables of nhi elements (D words each), table k at hi + k nhi D
+  const uint64_t* lo = nullptr;  // nt tables of 2^sh elements, table k at lo + k 2^sh D
+  uint32_t nt = 0;
+  uint32_t sh = 0;
+  uint64_t nhi = 0;
+};
+// Device words covview_build_dev needs for np eq terms and m geometric terms over 2^nv
+// elements.
+size_t covview_words(int D, size_t np, size_t m, size_t nv);
+// The view of covector_dev's sum (no accumulate) for n = 2^nv, split at sh = nv - nv/2:
+// np eq terms (points of nv elements, scalars), then m geometric terms; tables in buf.
+CovView covview_build_dev(int D, uint64_t* buf, size_t nv, const uint64_t* eq_points, const uint64_t* eq_scalars,
+                          size_t np, const uint64_t* scalars, const uint64_t* points, size_t m, hipStream_t st);
+// The view of y folded by r1 then r2 (sumcheck_pair_dev's double fold): hi tables of
+// nhi / 4 elements written to out_hi (nt nhi / 4 elements).  Needs nhi >= 4.
+CovView covview_fold2_dev(int D, const CovView& v, const uint64_t* r1, const uint64_t* r2, uint64_t* out_hi,
+                          hipStream_t st);
+// y(i) for i < n into out.
+void covview_write_dev(int D, const CovView& v, uint64_t* out, size_t n, hipStream_t st);
+// sumcheck_pair_dev with b given by a view: mode 0 reads y from v instead of b; modes
+// 1-3 take v as the view of the covector ALREADY folded by (r1, r2) (covview_fold2_dev)
+// and write its values into b (length L), so b holds the folded covector afterwards.
+void sumcheck_pair_view_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                            const uint64_t* r1, const uint64_t* r2, const CovView& v, uint64_t* dev_out,
+                            hipStream_t st);
+
 // eval_eq (src/algebra/multilinear.rs:170-190): acc[x] += s * prod_i eq(z_i, x_i), x_0 = MSB.
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t st);
 

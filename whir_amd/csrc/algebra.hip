@@ -448,9 +448,21 @@ __device__ __forceinline__ Ext<D> fold_twice(const uint64_t* v, size_t t, size_t
   return gl::eadd<D>(p, gl::emul<D>(gl::esub<D>(q, p), r2));
 }
 
-template <int Ds, int D, bool FOLD, int NS>
+// Element i of a covector view: sum_k hi_k[i >> sh] lo_k[i & (2^sh - 1)].
+template <int D>
+__device__ __forceinline__ Ext<D> view_at(const CovView& v, size_t i) {
+  const size_t h = i >> v.sh, l = i & (((size_t)1 << v.sh) - 1);
+  Ext<D> r = gl::emul<D>(ld<D>(v.hi, h), ld<D>(v.lo, l));
+  for (uint32_t k = 1; k < v.nt; k++)
+    r = gl::eadd<D>(r, gl::emul<D>(ld<D>(v.hi, (size_t)k * v.nhi + h), ld<D>(v.lo, ((size_t)k << v.sh) + l)));
+  return r;
+}
+
+// VIEW: b's values come from the view v (mode 0: y itself; folding modes: y already
+// folded by (r1, r2)) instead of from b, and the folding modes write them into b.
+template <int Ds, int D, bool FOLD, int NS, bool VIEW>
 __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
-                                                Ext<D> r1, Ext<D> r2, uint64_t* parts, MailboxSig sig) {
+                                                Ext<D> r1, Ext<D> r2, uint64_t* parts, MailboxSig sig, CovView v) {
   constexpr int NA = NS > 0 ? NS : 1;
   Ext<D> acc[NA];
 #pragma unroll
@@ -460,7 +472,8 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
     // the double fold alone: one output position per lane (four times the lanes of the
     // summing passes, which the short vectors at the end of a sumcheck need)
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < L; t += (size_t)gridDim.x * blockDim.x) {
-      const Ext<D> x = fold_twice<Ds, D>(a_in, t, L, r1, r2), y = fold_twice<D, D>(b, t, L, r1, r2);
+      const Ext<D> x = fold_twice<Ds, D>(a_in, t, L, r1, r2);
+      const Ext<D> y = VIEW ? view_at<D>(v, t) : fold_twice<D, D>(b, t, L, r1, r2);
       st<D>(a_out, t, x);
       st<D>(b, t, y);
     }
@@ -475,7 +488,7 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         x[k] = a_in[k * q + i];
-        y[k] = ld<D>(b, k * q + i);
+        y[k] = VIEW ? view_at<D>(v, k * q + i) : ld<D>(b, k * q + i);
       }
       const uint64_t dx0 = gl::sub(x[2], x[0]), dx1 = gl::sub(x[3], x[1]);
       const Ext<D> dy0 = gl::esub<D>(y[2], y[0]), dy1 = gl::esub<D>(y[3], y[1]);
@@ -501,10 +514,10 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
       const size_t t = k * q + i;
       if constexpr (FOLD) {
         x[k] = fold_twice<Ds, D>(a_in, t, L, r1, r2);
-        y[k] = fold_twice<D, D>(b, t, L, r1, r2);
+        y[k] = VIEW ? view_at<D>(v, t) : fold_twice<D, D>(b, t, L, r1, r2);
       } else {
         x[k] = ldx<Ds, D>(a_in, t);
-        y[k] = ld<D>(b, t);
+        y[k] = VIEW ? view_at<D>(v, t) : ld<D>(b, t);
       }
     }
     if constexpr (FOLD) {
@@ -535,9 +548,9 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
   if constexpr (NS > 0) block_reduce_store<D, NS>(acc, parts + (size_t)blockIdx.x * NS * D, sig);
 }
 
-template <int Ds, int D, bool FOLD, int NS>
+template <int Ds, int D, bool FOLD, int NS, bool VIEW>
 static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L, const uint64_t* r1,
-                      const uint64_t* r2, uint64_t* out, hipStream_t s) {
+                      const uint64_t* r2, uint64_t* out, const CovView& v, hipStream_t s) {
   const size_t q = L >> 2;
   const unsigned g = NS == 0 ? grid_for(L) : (q <= SMALL_PAIRS ? 1u : grid_for(q));
   DevBuf parts(NS > 0 && g > 1 ? (size_t)g * NS * D * 8 : 0, s);
@@ -546,8 +559,9 @@ static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t
     ProfScope prof(kernel_name(FOLD ? (NS ? "k_sc_pair_fold" : "k_fold4") : "k_sc_pair", Ds, D), s,
                    FOLD ? 8.0 * ((Ds + D) * 4.0 * L + 2.0 * D * L) : 8.0 * (Ds + D) * (double)L);
     const bool direct = NS > 0 && g == 1;  // one block stores the sums itself
-    k_sc_pair<Ds, D, FOLD, NS><<<g, RT, 0, s>>>(a_in, a_out, b, L, w1, w2, direct ? out : parts.as<uint64_t>(),
-                                                direct ? mailbox_take(out) : MailboxSig{});
+    k_sc_pair<Ds, D, FOLD, NS, VIEW><<<g, RT, 0, s>>>(a_in, a_out, b, L, w1, w2,
+                                                      direct ? out : parts.as<uint64_t>(),
+                                                      direct ? mailbox_take(out) : MailboxSig{}, v);
     WHIR_HIP(hipGetLastError());
   }
   if constexpr (NS > 0) {
@@ -555,33 +569,46 @@ static void sc_pair_t(const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t
   }
 }
 
-template <int Ds, int D>
+template <int Ds, int D, bool VIEW>
 static void sc_pair_d(int mode, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L, const uint64_t* r1,
-                      const uint64_t* r2, uint64_t* out, hipStream_t s) {
+                      const uint64_t* r2, uint64_t* out, const CovView& v, hipStream_t s) {
   switch (mode) {
-    case 0: sc_pair_t<Ds, D, false, 8>(a_in, a_out, b, L, r1, r2, out, s); return;
-    case 1: sc_pair_t<Ds, D, true, 8>(a_in, a_out, b, L, r1, r2, out, s); return;
-    case 2: sc_pair_t<Ds, D, true, 2>(a_in, a_out, b, L, r1, r2, out, s); return;
-    default: sc_pair_t<Ds, D, true, 0>(a_in, a_out, b, L, r1, r2, out, s); return;
+    case 0: sc_pair_t<Ds, D, false, 8, VIEW>(a_in, a_out, b, L, r1, r2, out, v, s); return;
+    case 1: sc_pair_t<Ds, D, true, 8, VIEW>(a_in, a_out, b, L, r1, r2, out, v, s); return;
+    case 2: sc_pair_t<Ds, D, true, 2, VIEW>(a_in, a_out, b, L, r1, r2, out, v, s); return;
+    default: sc_pair_t<Ds, D, true, 0, VIEW>(a_in, a_out, b, L, r1, r2, out, v, s); return;
   }
+}
+
+template <bool VIEW>
+static void sc_pair_any(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                        const uint64_t* r1, const uint64_t* r2, uint64_t* out, const CovView& v, hipStream_t s) {
+  WHIR_REQUIRE(L >= 4 && (L & (L - 1)) == 0, "sumcheck pair: length must be a power of two >= 4");
+  if (Ds == D) {
+    switch (D) {
+      case 1: sc_pair_d<1, 1, VIEW>(mode, a_in, a_out, b, L, r1, r2, out, v, s); return;
+      case 2: sc_pair_d<2, 2, VIEW>(mode, a_in, a_out, b, L, r1, r2, out, v, s); return;
+      case 3: sc_pair_d<3, 3, VIEW>(mode, a_in, a_out, b, L, r1, r2, out, v, s); return;
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: sc_pair_d<1, 2, VIEW>(mode, a_in, a_out, b, L, r1, r2, out, v, s); return;
+      case 3: sc_pair_d<1, 3, VIEW>(mode, a_in, a_out, b, L, r1, r2, out, v, s); return;
+    }
+  }
+  throw Error("sumcheck pair: extension degree must be 1, 2 or 3");
 }
 
 void sumcheck_pair_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
                        const uint64_t* r1, const uint64_t* r2, uint64_t* out, hipStream_t s) {
-  WHIR_REQUIRE(L >= 4 && (L & (L - 1)) == 0, "sumcheck pair: length must be a power of two >= 4");
-  if (Ds == D) {
-    switch (D) {
-      case 1: sc_pair_d<1, 1>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
-      case 2: sc_pair_d<2, 2>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
-      case 3: sc_pair_d<3, 3>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
-    }
-  } else if (Ds == 1) {
-    switch (D) {
-      case 2: sc_pair_d<1, 2>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
-      case 3: sc_pair_d<1, 3>(mode, a_in, a_out, b, L, r1, r2, out, s); return;
-    }
-  }
-  throw Error("sumcheck pair: extension degree must be 1, 2 or 3");
+  sc_pair_any<false>(mode, Ds, D, a_in, a_out, b, L, r1, r2, out, CovView{}, s);
+}
+
+void sumcheck_pair_view_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
+                            const uint64_t* r1, const uint64_t* r2, const CovView& v, uint64_t* out, hipStream_t s) {
+  WHIR_REQUIRE(v.nt > 0 && v.hi && v.lo, "sumcheck pair: empty covector view");
+  WHIR_REQUIRE(((size_t)v.nhi << v.sh) == L, "sumcheck pair: covector view length mismatch");
+  sc_pair_any<true>(mode, Ds, D, a_in, a_out, b, L, r1, r2, out, v, s);
 }
 
 // ------------------------------------------------------------------ covector terms
@@ -824,6 +851,155 @@ void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_
     case 3: covector_t<3>(acc, n, accumulate, eq_points, eq_scalars, np, nv, scalars, points, m, s); break;
     default: throw Error("covector: extension degree must be 1, 2 or 3");
   }
+}
+
+// ------------------------------------------------------------------ covector views
+// Table layout in the buffer: the uploaded inputs (when they do not travel with the
+// launch), then the nt hi tables, then the nt lo tables.  Terms: np eq terms (points
+// z_p of nv elements, scalars s_p), then m geometric terms (points x_j, scalars t_j):
+//   eq:  hi[h] = s_p prod_{q < nv-sh} (h_q ? z_q : 1 - z_q)   (h_0 = MSB of h)
+//        lo[l] = prod_{q < sh} (l_q ? z_(nv-sh+q) : 1 - z_(nv-sh+q))
+//   geo: hi[h] = t_j x_j^(h 2^sh), lo[l] = x_j^l.
+struct CovViewPlan {
+  size_t np, m, nv, sh, nhi, nlo;
+  size_t in_z, in_s, in_x, in_t;  // input offsets (words)
+  size_t hi, lo;                  // table offsets (words)
+};
+
+template <int D>
+__global__ void k_covview_tables(uint64_t* buf, CovViewPlan L, const uint64_t* in_dev, CovIn in_arg) {
+  size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t nt = L.np + L.m, k1 = L.nv - L.sh;
+  if (e < nt * L.nhi) {
+    const size_t k = e / L.nhi, h = e % L.nhi;
+    Ext<D> v;
+    if (k < L.np) {
+      v = in_ld<D>(in_dev, in_arg, L.in_s, k);
+      for (size_t q = 0; q < k1; q++) {
+        const Ext<D> zq = in_ld<D>(in_dev, in_arg, L.in_z, k * L.nv + q);
+        v = gl::emul<D>(v, ((h >> (k1 - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
+      }
+    } else {
+      const size_t j = k - L.np;
+      v = gl::emul<D>(in_ld<D>(in_dev, in_arg, L.in_t, j),
+                      gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, j), (uint64_t)h << L.sh));
+    }
+    st<D>(buf + L.hi, e, v);
+    return;
+  }
+  e -= nt * L.nhi;
+  if (e >= nt * L.nlo) return;
+  const size_t k = e / L.nlo, l = e % L.nlo;
+  Ext<D> v;
+  if (k < L.np) {
+    v = gl::ext_base<D>(1);
+    for (size_t q = 0; q < L.sh; q++) {
+      const Ext<D> zq = in_ld<D>(in_dev, in_arg, L.in_z, k * L.nv + k1 + q);
+      v = gl::emul<D>(v, ((l >> (L.sh - 1 - q)) & 1) ? zq : gl::esub<D>(gl::ext_base<D>(1), zq));
+    }
+  } else {
+    v = gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, k - L.np), l);
+  }
+  st<D>(buf + L.lo, e, v);
+}
+
+size_t covview_words(int D, size_t np, size_t m, size_t nv) {
+  const size_t sh = nv - nv / 2, nt = np + m;
+  return (np * nv + np + 2 * m) * D + nt * (((size_t)1 << (nv - sh)) + ((size_t)1 << sh)) * D;
+}
+
+template <int D>
+static CovView covview_build_t(uint64_t* buf, size_t nv, const uint64_t* eq_points, const uint64_t* eq_scalars,
+                               size_t np, const uint64_t* scalars, const uint64_t* points, size_t m, hipStream_t s) {
+  CovViewPlan L{};
+  L.np = np;
+  L.m = m;
+  L.nv = nv;
+  L.sh = nv - nv / 2;
+  L.nhi = (size_t)1 << (nv - L.sh);
+  L.nlo = (size_t)1 << L.sh;
+  std::vector<uint64_t> in;
+  L.in_z = 0;
+  in.insert(in.end(), eq_points, eq_points + np * nv * D);
+  L.in_s = in.size();
+  in.insert(in.end(), eq_scalars, eq_scalars + np * D);
+  L.in_x = in.size();
+  in.insert(in.end(), points, points + m * D);
+  L.in_t = in.size();
+  in.insert(in.end(), scalars, scalars + m * D);
+  const bool by_arg = in.size() <= COV_ARG_WORDS;
+  const size_t nt = np + m;
+  L.hi = in.size();  // the input region stays reserved either way (covview_words)
+  L.lo = L.hi + nt * L.nhi * D;
+  CovIn args{};
+  if (by_arg) std::copy(in.begin(), in.end(), args.w);
+  else upload_async(buf, in.data(), in.size() * 8, s);
+  const size_t entries = nt * (L.nhi + L.nlo);
+  k_covview_tables<D><<<blocks_for(entries, RT), RT, 0, s>>>(buf, L, by_arg ? nullptr : buf, args);
+  WHIR_HIP(hipGetLastError());
+  CovView v;
+  v.hi = buf + L.hi;
+  v.lo = buf + L.lo;
+  v.nt = (uint32_t)nt;
+  v.sh = (uint32_t)L.sh;
+  v.nhi = L.nhi;
+  return v;
+}
+
+CovView covview_build_dev(int D, uint64_t* buf, size_t nv, const uint64_t* eq_points, const uint64_t* eq_scalars,
+                          size_t np, const uint64_t* scalars, const uint64_t* points, size_t m, hipStream_t s) {
+  WHIR_REQUIRE(np + m > 0 && nv >= 2 && nv < 63, "covector view: needs terms and 2 <= nv < 63");
+  switch (D) {
+    case 1: return covview_build_t<1>(buf, nv, eq_points, eq_scalars, np, scalars, points, m, s);
+    case 2: return covview_build_t<2>(buf, nv, eq_points, eq_scalars, np, scalars, points, m, s);
+    case 3: return covview_build_t<3>(buf, nv, eq_points, eq_scalars, np, scalars, points, m, s);
+    default: throw Error("covector view: extension degree must be 1, 2 or 3");
+  }
+}
+
+template <int D>
+__global__ void k_covview_fold2(const uint64_t* hi, uint64_t* out, size_t nt, size_t nhi, Ext<D> r1, Ext<D> r2) {
+  const size_t q = nhi >> 2;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= nt * q) return;
+  const size_t k = e / q, t = e % q;
+  st<D>(out, e, fold_twice<D, D>(hi + k * nhi * D, t, q, r1, r2));
+}
+
+template <int D>
+__global__ void k_covview_write(CovView v, uint64_t* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    st<D>(out, i, view_at<D>(v, i));
+}
+
+CovView covview_fold2_dev(int D, const CovView& v, const uint64_t* r1, const uint64_t* r2, uint64_t* out_hi,
+                          hipStream_t s) {
+  WHIR_REQUIRE(v.nt > 0 && v.nhi >= 4, "covector view: nothing left to fold in the high tables");
+  const size_t n = (size_t)v.nt * (v.nhi >> 2);
+  switch (D) {
+    case 1: k_covview_fold2<1><<<blocks_for(n, RT), RT, 0, s>>>(v.hi, out_hi, v.nt, v.nhi, host_ext<1>(r1), host_ext<1>(r2)); break;
+    case 2: k_covview_fold2<2><<<blocks_for(n, RT), RT, 0, s>>>(v.hi, out_hi, v.nt, v.nhi, host_ext<2>(r1), host_ext<2>(r2)); break;
+    case 3: k_covview_fold2<3><<<blocks_for(n, RT), RT, 0, s>>>(v.hi, out_hi, v.nt, v.nhi, host_ext<3>(r1), host_ext<3>(r2)); break;
+    default: throw Error("covector view: extension degree must be 1, 2 or 3");
+  }
+  WHIR_HIP(hipGetLastError());
+  CovView f = v;
+  f.hi = out_hi;
+  f.nhi = v.nhi >> 2;
+  return f;
+}
+
+void covview_write_dev(int D, const CovView& v, uint64_t* out, size_t n, hipStream_t s) {
+  WHIR_REQUIRE(n <= ((size_t)v.nhi << v.sh), "covector view: output longer than the view");
+  if (!n) return;
+  const unsigned g = grid_for(n);
+  switch (D) {
+    case 1: k_covview_write<1><<<g, RT, 0, s>>>(v, out, n); break;
+    case 2: k_covview_write<2><<<g, RT, 0, s>>>(v, out, n); break;
+    case 3: k_covview_write<3><<<g, RT, 0, s>>>(v, out, n); break;
+    default: throw Error("covector view: extension degree must be 1, 2 or 3");
+  }
+  WHIR_HIP(hipGetLastError());
 }
 
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t s) {

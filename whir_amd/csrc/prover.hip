@@ -15,6 +15,7 @@
 // Every vector of the protocol (lifted vector, covector, codewords, Merkle trees)
 // lives in HBM for the whole proof; the host only sees the transcript values:
 // roots, OOD and sumcheck messages, opened rows and sibling hashes.
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -131,6 +132,17 @@ void pow_prove(Transcript& t, int engine, uint64_t threshold, hipStream_t s) {
     WHIR_REQUIRE(first <= UINT64_MAX - 2 * window, "Proof of Work failed to solve.");
   }
   t.prover_u64(nonce);
+}
+
+// Covector views (see CovView) for initial covectors of at most this many terms;
+// WHIR_COV_VIEW=0 always materializes the covector (for comparison).
+constexpr size_t COV_VIEW_MAX_TERMS = 4;
+static bool cov_view_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("WHIR_COV_VIEW");
+    return !e || atoi(e) != 0;
+  }();
+  return on;
 }
 
 int log2_exact(uint64_t n) {
@@ -436,10 +448,27 @@ void irs_open_finish(Transcript& t, std::vector<OpenLaunch>& ls, std::vector<std
 // host round trip of its own; the two challenges are folded into the vectors by the
 // next pass.  The messages, challenges and folded vectors are the ones the round-by-
 // round loop below makes.
+//
+// view, when given, is b as a covector view (b's contents are then unused): the first
+// pass reads it, the first fold folds its high tables and writes the folded values
+// into b, and any other first use of b writes the view's values into b first.
 template <int D>
 std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src,
                                        int a_Ds, uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
-                                       const std::function<void()>& after_first) {
+                                       const std::function<void()>& after_first, const CovView* view) {
+  CovView cur = view ? *view : CovView{};  // cur.nt == 0: b holds the covector
+  DevBuf folded_hi;
+  // The view folded by the pending pair, for a pass that folds (b is written by it).
+  auto fold_view = [&](const E<D>& r1, const E<D>& r2) {
+    folded_hi = DevBuf((size_t)cur.nt * (cur.nhi >> 2) * D * 8, s);
+    const CovView f = covview_fold2_dev(D, cur, r1.c, r2.c, folded_hi.as<uint64_t>(), s);
+    cur = CovView{};
+    return f;
+  };
+  auto materialize = [&] {
+    if (cur.nt) covview_write_dev(D, cur, b, (size_t)cur.nhi << cur.sh, s);  // still unfolded
+    cur = CovView{};
+  };
   bool first = true;
   auto synced = [&] {
     mailbox_wait(s);
@@ -465,9 +494,15 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     if (sc.num_rounds - round >= 2 && len >= 16 && pend.size() != 1) {
       // round k's polynomial and round k+1's coefficients, folding the pending pair first
       if (pend.empty()) {
-        sumcheck_pair_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, mbox, s);
+        if (cur.nt) sumcheck_pair_view_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, cur, mbox, s);
+        else sumcheck_pair_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, mbox, s);
       } else {
-        sumcheck_pair_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        if (cur.nt) {
+          const CovView f = fold_view(pend[0], pend[1]);
+          sumcheck_pair_view_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, mbox, s);
+        } else {
+          sumcheck_pair_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        }
         a_src = a;
         a_Ds = D;
       }
@@ -487,14 +522,21 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     } else {
       // one round: its polynomial after folding whatever is pending
       if (pend.size() == 2) {
-        sumcheck_pair_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        if (cur.nt) {
+          const CovView f = fold_view(pend[0], pend[1]);
+          sumcheck_pair_view_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, mbox, s);
+        } else {
+          sumcheck_pair_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
+        }
         a_src = a;
         a_Ds = D;
       } else if (pend.size() == 1) {
+        materialize();
         fold_and_poly_mixed_dev(a_Ds, D, a_src, a, b, 2 * len, pend[0].c, mbox, s);
         a_src = a;
         a_Ds = D;
       } else {
+        materialize();
         sumcheck_poly_mixed_dev(a_Ds, D, a_src, b, len, mbox, s);
       }
       synced();
@@ -506,11 +548,18 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     }
   }
   if (pend.size() == 2) {
-    sumcheck_pair_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, nullptr, s);
+    if (cur.nt) {
+      const CovView f = fold_view(pend[0], pend[1]);
+      sumcheck_pair_view_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, nullptr, s);
+    } else {
+      sumcheck_pair_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, nullptr, s);
+    }
   } else if (pend.size() == 1) {
+    materialize();
     fold_both_dev(a_Ds, D, a_src, a, b, 2 * len, pend[0].c, s);
-  } else if (a_src != a) {
-    lift_dev(D, a_src, len, a, s);
+  } else {
+    materialize();
+    if (a_src != a) lift_dev(D, a_src, len, a, s);
   }
   n = len;
   return chals;
@@ -520,13 +569,16 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
 // after_first, when given, runs once after the first synchronization (before the first
 // message is sent), or at the end when there are no rounds: a caller that queued
 // work ahead of the sumcheck (an opening's gather) reads its results there.
+//
+// view: b given as a covector view of length n (see sumcheck_prove_pairs).
 template <int D>
 std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src, int a_Ds,
                                  uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
-                                 const std::function<void()>& after_first = {}) {
+                                 const std::function<void()>& after_first = {}, const CovView* view = nullptr) {
   WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
   if (n >= 16 && (n & (n - 1)) == 0 && sc.num_rounds >= 2)
-    return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s, after_first);
+    return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s, after_first, view);
+  if (view) covview_write_dev(D, *view, b, n, s);
   bool first = true;
   std::vector<E<D>> chals;
   uint64_t* mbox = result_mailbox(s);
@@ -932,6 +984,12 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
   };
   std::vector<E<D>> crlc = geometric_challenge<D>(t, np + nood);
   const bool has_constraints = !crlc.empty();
+  // The initial covector is kept as a view (a few product terms in small tables) when it
+  // has no Covector forms: the first sumcheck pass reads it from the tables instead of
+  // from 2^nv values in HBM, and the first fold writes the folded values.
+  bool use_view = false;
+  DevBuf view_buf;
+  CovView view;
   {
     // cov = sum_{MLE forms k} crlc_k eq(point_k, .) + sum_o crlc_(np+o) (1, x_o, x_o^2, ...)
     // in one pass, then + crlc_k covector_k for the Covector forms (Covector::accumulate,
@@ -959,10 +1017,19 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     }
     for (size_t k = 0; k < nood; k++) push<D>(srlc, crlc[np + k]);
     geo_pts.insert(geo_pts.end(), opts.begin(), opts.end());
-    covector_dev(D, cov.as<uint64_t>(), n, false, mle_pts.data(), erlc.data(), nmle, nv, srlc.data(),
-                 geo_pts.data(), srlc.size() / D, s);
-    for (size_t q = 0; q < cov_forms.size(); q++)
-      scalar_mul_add_dev(D, cov.as<uint64_t>(), crlc[cov_forms[q]].c, covs[q], n, s);
+    const size_t ngeo = srlc.size() / D;
+    use_view = cov_view_enabled() && cov_forms.empty() && nmle + ngeo > 0 && nmle + ngeo <= COV_VIEW_MAX_TERMS &&
+               n == ((size_t)1 << nv) && nv >= 4;
+    if (use_view) {
+      view_buf = DevBuf(covview_words(D, nmle, ngeo, nv) * 8, s);
+      view = covview_build_dev(D, view_buf.as<uint64_t>(), nv, mle_pts.data(), erlc.data(), nmle, srlc.data(),
+                               geo_pts.data(), ngeo, s);
+    } else {
+      covector_dev(D, cov.as<uint64_t>(), n, false, mle_pts.data(), erlc.data(), nmle, nv, srlc.data(),
+                   geo_pts.data(), ngeo, s);
+      for (size_t q = 0; q < cov_forms.size(); q++)
+        scalar_mul_add_dev(D, cov.as<uint64_t>(), crlc[cov_forms[q]].c, covs[q], n, s);
+    }
   }
   E<D> the_sum = gl::ext_zero<D>();
   {
@@ -976,7 +1043,7 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
   std::vector<E<D>> fr;
   if (has_constraints) {
     fr = sumcheck_prove<D>(t, cfg.initial_sumcheck, eng, a0, a0_Ds, vec.as<uint64_t>(), cov.as<uint64_t>(), n,
-                           the_sum, s);
+                           the_sum, s, {}, use_view ? &view : nullptr);
   } else {
     for (size_t i = 0; i < cfg.initial_sumcheck.num_rounds; i++) fr.push_back(verifier_field<D>(t));
     pow_prove(t, eng, cfg.initial_skip_pow, s);
