@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace whir {
@@ -34,12 +35,11 @@ __device__ __forceinline__ void st(uint64_t* p, size_t i, const Ext<D>& v) {
 template <int D>
 __device__ __forceinline__ Ext<D> ld_host(const uint64_t* const* unused, const Ext<D>& v) { return v; }
 
-// Sum NV extension values over the block; thread 0 writes them to out (and then raises
-// the mailbox flag when sig asks it to).  Each wave sums its lanes by cross-lane
-// exchanges (no barrier), its first lane puts the wave's sum in LDS, and after one
-// barrier thread 0 adds the waves' sums.
+// Sum NV extension values over the block into thread 0's v (other threads' v are left
+// partial).  Each wave sums its lanes by cross-lane exchanges (no barrier), its first
+// lane puts the wave's sum in LDS, and after one barrier thread 0 adds the waves' sums.
 template <int D, int NV>
-__device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out, MailboxSig sig = {}) {
+__device__ void block_sum(Ext<D> v[NV]) {
   __shared__ uint64_t red[RT / 64][NV * D];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1)
@@ -61,18 +61,28 @@ __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out, MailboxSig sig =
     const int waves = blockDim.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-      Ext<D> acc;
 #pragma unroll
-      for (int d = 0; d < D; d++) acc.c[d] = red[0][k * D + d];
+      for (int d = 0; d < D; d++) v[k].c[d] = red[0][k * D + d];
       for (int w = 1; w < waves; w++) {
         Ext<D> x;
 #pragma unroll
         for (int d = 0; d < D; d++) x.c[d] = red[w][k * D + d];
-        acc = gl::eadd<D>(acc, x);
+        v[k] = gl::eadd<D>(v[k], x);
       }
-#pragma unroll
-      for (int d = 0; d < D; d++) out[k * D + d] = acc.c[d];
     }
+  }
+}
+
+// Sum NV extension values over the block; thread 0 writes them to out (and then raises
+// the mailbox flag when sig asks it to).
+template <int D, int NV>
+__device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out, MailboxSig sig = {}) {
+  block_sum<D, NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+      for (int d = 0; d < D; d++) out[k * D + d] = v[k].c[d];
     mailbox_raise(sig);
   }
 }
@@ -604,11 +614,175 @@ void sumcheck_pair_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* 
   sc_pair_any<false>(mode, Ds, D, a_in, a_out, b, L, r1, r2, out, CovView{}, s);
 }
 
+// ------------------------------------------------------------------ view passes
+// The sums of a pairs pass whose covector y is a view, factored by the view's blocks.
+// With quarters x_a(i) = x(a L/4 + i), y_b likewise (i < L/4), every sum of the pass is
+// a fixed bilinear combination of the 16 values G[a][b] = sum_i x_a(i) y_b(i), and
+//   G[a][b] = sum_t sum_h hi_t[b Qb + h] P_(a,t)(h),  P_(a,t)(h) = sum_l x_a(h 2^sh + l) lo_t[l]
+// (Qb = L/4 / 2^sh blocks per quarter): the inner products run against the low tables
+// (a base-field x costs D products per term, summed as 128-bit values and reduced once
+// per block), and the high tables enter once per block.  One GPU block per (a, h).
+constexpr int VIEW_MAX_TERMS = 4;
+
+template <int Ds, int D, bool FOLD, int NT>
+__global__ __launch_bounds__(RT) void k_view_pass(const uint64_t* a_in, uint64_t* a_out, size_t L, Ext<D> r1,
+                                                  Ext<D> r2, CovView v, uint64_t* parts) {
+  const size_t q = L >> 2, B = (size_t)1 << v.sh, Qb = q >> v.sh;
+  const size_t a = blockIdx.x / Qb, h = blockIdx.x % Qb;
+  const size_t base = a * q + h * B;
+  Ext<D> P[NT];
+  if constexpr (!FOLD && Ds == 1) {
+    gl::Wide W[NT][D];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int d = 0; d < D; d++) W[t][d] = gl::Wide{0, 0, 0};
+#pragma unroll 4
+    for (size_t l = threadIdx.x; l < B; l += blockDim.x) {
+      const uint64_t x = a_in[base + l];
+      Ext<D> lo[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) lo[t] = ld<D>(v.lo, ((size_t)t << v.sh) + l);
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int d = 0; d < D; d++) gl::wide_mac(W[t][d], x, lo[t].c[d]);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int d = 0; d < D; d++) P[t].c[d] = gl::wide_reduce(W[t][d]);
+  } else {
+#pragma unroll
+    for (int t = 0; t < NT; t++) P[t] = gl::ext_zero<D>();
+#pragma unroll 2
+    for (size_t l = threadIdx.x; l < B; l += blockDim.x) {
+      const size_t p = base + l;
+      Ext<D> lo[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) lo[t] = ld<D>(v.lo, ((size_t)t << v.sh) + l);
+      Ext<D> x;
+      if constexpr (FOLD) {
+        x = fold_twice<Ds, D>(a_in, p, L, r1, r2);
+        st<D>(a_out, p, x);
+      } else {
+        x = ldx<Ds, D>(a_in, p);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) P[t] = gl::eadd<D>(P[t], gl::emul<D>(x, lo[t]));
+    }
+  }
+  block_sum<D, NT>(P);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      Ext<D> g = gl::ext_zero<D>();
+#pragma unroll
+      for (int t = 0; t < NT; t++) g = gl::eadd<D>(g, gl::emul<D>(ld<D>(v.hi, (size_t)t * v.nhi + b * Qb + h), P[t]));
+      st<D>(parts, (size_t)blockIdx.x * 4 + b, g);
+    }
+  }
+}
+
+// B(al, be) = sum_ab al_a be_b G[a][b] for coefficient vectors in {-1, 0, 1}.
+template <int D>
+__device__ Ext<D> view_bilinear(const Ext<D> (&G)[16], const int (&al)[4], const int (&be)[4]) {
+  Ext<D> r = gl::ext_zero<D>();
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int c = al[a] * be[b];
+      if (c > 0) r = gl::eadd<D>(r, G[a * 4 + b]);
+      else if (c < 0) r = gl::esub<D>(r, G[a * 4 + b]);
+    }
+  return r;
+}
+
+// Sums the blocks' G contributions and writes the pass's NS sums, the same values the
+// pairs kernel's accumulators hold (see k_sc_pair: quarters 0..3 of x and y, dx0 = x2 - x0,
+// dx1 = x3 - x1, ux = x1 - x0, vx = dx1 - dx0, likewise for y).
+template <int D, int NS>
+__global__ __launch_bounds__(RT) void k_view_reduce(const uint64_t* parts, size_t Qb, uint64_t* out, MailboxSig sig) {
+  Ext<D> G[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) G[k] = gl::ext_zero<D>();
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+    for (size_t h = threadIdx.x; h < Qb; h += blockDim.x)
+#pragma unroll
+      for (int b = 0; b < 4; b++) G[a * 4 + b] = gl::eadd<D>(G[a * 4 + b], ld<D>(parts, (a * Qb + h) * 4 + b));
+  block_sum<D, 16>(G);
+  if (threadIdx.x != 0) return;
+  const int e0[4] = {1, 0, 0, 0}, e1[4] = {0, 1, 0, 0}, d0[4] = {-1, 0, 1, 0}, d1[4] = {0, -1, 0, 1},
+            u[4] = {-1, 1, 0, 0}, w[4] = {1, -1, -1, 1};
+  Ext<D> acc[8];
+  acc[0] = gl::eadd<D>(view_bilinear<D>(G, e0, e0), view_bilinear<D>(G, e1, e1));  // x0 y0 + x1 y1
+  acc[1] = gl::eadd<D>(view_bilinear<D>(G, d0, d0), view_bilinear<D>(G, d1, d1));  // dx0 dy0 + dx1 dy1
+  if constexpr (NS == 8) {
+    acc[2] = view_bilinear<D>(G, e0, e0);                                                // x0 y0
+    acc[3] = gl::eadd<D>(view_bilinear<D>(G, e0, d0), view_bilinear<D>(G, d0, e0));  // x0 dy0 + dx0 y0
+    acc[4] = view_bilinear<D>(G, d0, d0);                                                // dx0 dy0
+    acc[5] = view_bilinear<D>(G, u, u);                                                  // ux uy
+    acc[6] = gl::eadd<D>(view_bilinear<D>(G, u, w), view_bilinear<D>(G, w, u));        // ux vy + vx uy
+    acc[7] = view_bilinear<D>(G, w, w);                                                  // vx vy
+  }
+#pragma unroll
+  for (int k = 0; k < NS; k++) st<D>(out, k, acc[k]);
+  mailbox_raise(sig);
+}
+
+template <int Ds, int D>
+static void view_pass_t(int mode, const uint64_t* a_in, uint64_t* a_out, size_t L, const uint64_t* r1,
+                        const uint64_t* r2, const CovView& v, uint64_t* out, hipStream_t s) {
+  const size_t Qb = (L >> 2) >> v.sh;
+  const unsigned blocks = (unsigned)(4 * Qb);
+  DevBuf parts((size_t)blocks * 4 * D * 8, s);
+  const Ext<D> w1 = r1 ? host_ext<D>(r1) : gl::ext_zero<D>(), w2 = r2 ? host_ext<D>(r2) : gl::ext_zero<D>();
+  {
+    ProfScope prof(kernel_name(mode ? "k_view_pass_fold" : "k_view_pass", Ds, D), s,
+                   mode ? 8.0 * (Ds * 4.0 * L + D * (double)L) : 8.0 * Ds * (double)L);
+    uint64_t* pp = parts.as<uint64_t>();
+    auto go = [&](auto nt) {
+      constexpr int NT = decltype(nt)::value;
+      if (mode == 0) k_view_pass<Ds, D, false, NT><<<blocks, RT, 0, s>>>(a_in, a_out, L, w1, w2, v, pp);
+      else k_view_pass<Ds, D, true, NT><<<blocks, RT, 0, s>>>(a_in, a_out, L, w1, w2, v, pp);
+    };
+    switch (v.nt) {
+      case 1: go(std::integral_constant<int, 1>{}); break;
+      case 2: go(std::integral_constant<int, 2>{}); break;
+      case 3: go(std::integral_constant<int, 3>{}); break;
+      default: go(std::integral_constant<int, 4>{}); break;
+    }
+    WHIR_HIP(hipGetLastError());
+  }
+  if (mode == 2) k_view_reduce<D, 2><<<1, RT, 0, s>>>(parts.as<uint64_t>(), Qb, out, mailbox_take(out));
+  else k_view_reduce<D, 8><<<1, RT, 0, s>>>(parts.as<uint64_t>(), Qb, out, mailbox_take(out));
+  WHIR_HIP(hipGetLastError());
+}
+
 void sumcheck_pair_view_dev(int mode, int Ds, int D, const uint64_t* a_in, uint64_t* a_out, uint64_t* b, size_t L,
                             const uint64_t* r1, const uint64_t* r2, const CovView& v, uint64_t* out, hipStream_t s) {
-  WHIR_REQUIRE(v.nt > 0 && v.hi && v.lo, "sumcheck pair: empty covector view");
+  WHIR_REQUIRE(v.nt > 0 && v.nt <= VIEW_MAX_TERMS && v.hi && v.lo, "sumcheck pair: bad covector view");
   WHIR_REQUIRE(((size_t)v.nhi << v.sh) == L, "sumcheck pair: covector view length mismatch");
-  sc_pair_any<true>(mode, Ds, D, a_in, a_out, b, L, r1, r2, out, v, s);
+  if (mode == 3) {  // the double fold alone; it writes the view's values into b
+    sc_pair_any<true>(mode, Ds, D, a_in, a_out, b, L, r1, r2, out, v, s);
+    return;
+  }
+  WHIR_REQUIRE(v.nhi >= 4 && mode >= 0 && mode <= 2, "sumcheck pair: covector view too short for a view pass");
+  if (Ds == D) {
+    switch (D) {
+      case 1: return view_pass_t<1, 1>(mode, a_in, a_out, L, r1, r2, v, out, s);
+      case 2: return view_pass_t<2, 2>(mode, a_in, a_out, L, r1, r2, v, out, s);
+      case 3: return view_pass_t<3, 3>(mode, a_in, a_out, L, r1, r2, v, out, s);
+    }
+  } else if (Ds == 1) {
+    switch (D) {
+      case 2: return view_pass_t<1, 2>(mode, a_in, a_out, L, r1, r2, v, out, s);
+      case 3: return view_pass_t<1, 3>(mode, a_in, a_out, L, r1, r2, v, out, s);
+    }
+  }
+  throw Error("sumcheck pair: extension degree must be 1, 2 or 3");
 }
 
 // ------------------------------------------------------------------ covector terms

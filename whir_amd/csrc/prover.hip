@@ -449,22 +449,26 @@ void irs_open_finish(Transcript& t, std::vector<OpenLaunch>& ls, std::vector<std
 // next pass.  The messages, challenges and folded vectors are the ones the round-by-
 // round loop below makes.
 //
-// view, when given, is b as a covector view (b's contents are then unused): the first
-// pass reads it, the first fold folds its high tables and writes the folded values
-// into b, and any other first use of b writes the view's values into b first.
+// view, when given, is b as a covector view (b's contents are then unused): pairs passes
+// read it (sumcheck_pair_view_dev), each double fold folds its high tables, and b gets
+// the view's values when the view ends (the last double fold writes them, any other
+// use of b writes them first).
 template <int D>
 std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src,
                                        int a_Ds, uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
                                        const std::function<void()>& after_first, const CovView* view) {
   CovView cur = view ? *view : CovView{};  // cur.nt == 0: b holds the covector
-  DevBuf folded_hi;
-  // The view folded by the pending pair, for a pass that folds (b is written by it).
+  DevBuf folded_hi[2];                      // high tables of the folded views, in turn
+  int fh = 0;
+  // The view folded by the pending pair (the pass that folds the vectors uses it).
   auto fold_view = [&](const E<D>& r1, const E<D>& r2) {
-    folded_hi = DevBuf((size_t)cur.nt * (cur.nhi >> 2) * D * 8, s);
-    const CovView f = covview_fold2_dev(D, cur, r1.c, r2.c, folded_hi.as<uint64_t>(), s);
-    cur = CovView{};
+    folded_hi[fh] = DevBuf((size_t)cur.nt * (cur.nhi >> 2) * D * 8, s);
+    const CovView f = covview_fold2_dev(D, cur, r1.c, r2.c, folded_hi[fh].as<uint64_t>(), s);
+    fh ^= 1;
     return f;
   };
+  // A view pass needs four high-table blocks per quarter (two more bits to fold).
+  auto view_ok = [&](size_t folds) { return cur.nt && (cur.nhi >> (2 * folds)) >= 4; };
   auto materialize = [&] {
     if (cur.nt) covview_write_dev(D, cur, b, (size_t)cur.nhi << cur.sh, s);  // still unfolded
     cur = CovView{};
@@ -493,13 +497,14 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     mailbox_arm(s);  // the pass below ends with its sums in the mailbox
     if (sc.num_rounds - round >= 2 && len >= 16 && pend.size() != 1) {
       // round k's polynomial and round k+1's coefficients, folding the pending pair first
+      if (cur.nt && !view_ok(pend.empty() ? 0 : 1)) materialize();
       if (pend.empty()) {
         if (cur.nt) sumcheck_pair_view_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, cur, mbox, s);
         else sumcheck_pair_dev(0, a_Ds, D, a_src, a, b, len, nullptr, nullptr, mbox, s);
       } else {
         if (cur.nt) {
-          const CovView f = fold_view(pend[0], pend[1]);
-          sumcheck_pair_view_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, mbox, s);
+          cur = fold_view(pend[0], pend[1]);
+          sumcheck_pair_view_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, cur, mbox, s);
         } else {
           sumcheck_pair_dev(1, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
         }
@@ -522,9 +527,10 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     } else {
       // one round: its polynomial after folding whatever is pending
       if (pend.size() == 2) {
+        if (cur.nt && !view_ok(1)) materialize();
         if (cur.nt) {
-          const CovView f = fold_view(pend[0], pend[1]);
-          sumcheck_pair_view_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, mbox, s);
+          cur = fold_view(pend[0], pend[1]);
+          sumcheck_pair_view_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, cur, mbox, s);
         } else {
           sumcheck_pair_dev(2, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, mbox, s);
         }
@@ -548,9 +554,11 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
     }
   }
   if (pend.size() == 2) {
+    if (cur.nt && cur.nhi < 4) materialize();
     if (cur.nt) {
       const CovView f = fold_view(pend[0], pend[1]);
-      sumcheck_pair_view_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, nullptr, s);
+      cur = CovView{};
+      sumcheck_pair_view_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, f, nullptr, s);  // writes b
     } else {
       sumcheck_pair_dev(3, a_Ds, D, a_src, a, b, len, pend[0].c, pend[1].c, nullptr, s);
     }
