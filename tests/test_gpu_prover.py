@@ -75,6 +75,11 @@ CASES = [
     # base-field run of paired rounds
     (16, 2, W.HASH_BLAKE3, 5, 5, 1, 1),
     (17, 1, W.HASH_BLAKE3, 4, 4, 1, 1),
+    # initial covector views: too few high-table bits for a second double fold (the view
+    # is written out mid-sumcheck), and five terms (four points and an OOD sample: the
+    # covector is materialized from the start)
+    (7, 2, W.HASH_BLAKE3, 4, 2, 1, 1),
+    (12, 2, W.HASH_BLAKE3, 4, 4, 1, 4),
 ]
 
 

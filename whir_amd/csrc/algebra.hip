@@ -87,6 +87,54 @@ __device__ void block_reduce_store(Ext<D> v[NV], uint64_t* out, MailboxSig sig =
   }
 }
 
+// Sum NV extension values over a block of RT threads through LDS: each thread stores
+// its values, 16 threads per value add 16 of the stored rows each, and one thread per
+// value adds those 16 partials.  The totals are in tot[0..NV) after the call (all
+// threads).  For single-block reductions of many values, where the cross-lane sum of
+// block_sum is a long chain of exchanges per value.
+template <int D, int NV>
+__device__ void block_sum_lds(const Ext<D> (&v)[NV], Ext<D> (&tot)[NV]) {
+  static_assert(NV * 16 <= RT, "block_sum_lds: at most RT / 16 values");
+  __shared__ uint64_t rows[RT][NV * D];
+  __shared__ uint64_t part[16][NV * D];
+  __shared__ uint64_t sum[NV * D];
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) rows[threadIdx.x][k * D + d] = v[k].c[d];
+  __syncthreads();
+  if (threadIdx.x < NV * 16) {
+    const int k = threadIdx.x >> 4, p = threadIdx.x & 15;
+    Ext<D> acc = gl::ext_zero<D>();
+    for (int r = p; r < (int)blockDim.x; r += 16) {
+      Ext<D> x;
+#pragma unroll
+      for (int d = 0; d < D; d++) x.c[d] = rows[r][k * D + d];
+      acc = gl::eadd<D>(acc, x);
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) part[p][k * D + d] = acc.c[d];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    Ext<D> acc = gl::ext_zero<D>();
+#pragma unroll
+    for (int p = 0; p < 16; p++) {
+      Ext<D> x;
+#pragma unroll
+      for (int d = 0; d < D; d++) x.c[d] = part[p][threadIdx.x * D + d];
+      acc = gl::eadd<D>(acc, x);
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) sum[threadIdx.x * D + d] = acc.c[d];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) tot[k].c[d] = sum[k * D + d];
+}
+
 // Final reduction of `nparts` block partials of NV*D words each into `out`
 // (device memory, or the stream's pinned mailbox).  A second launch rather than a
 // last-block-done reduction in the first: on CDNA4 the per-block device-scope
@@ -101,7 +149,13 @@ __global__ __launch_bounds__(RT) void k_reduce_parts(const uint64_t* parts, size
   for (size_t p = threadIdx.x; p < nparts; p += blockDim.x)
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = gl::eadd<D>(v[k], ld<D>(parts + p * NV * D, k));
-  block_reduce_store<D, NV>(v, out, sig);
+  Ext<D> tot[NV];
+  block_sum_lds<D, NV>(v, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) st<D>(out, k, tot[k]);
+    mailbox_raise(sig);
+  }
 }
 
 // last: this launch ends the work a mailbox wait may be armed for.
@@ -684,21 +738,6 @@ __global__ __launch_bounds__(RT) void k_view_pass(const uint64_t* a_in, uint64_t
   }
 }
 
-// B(al, be) = sum_ab al_a be_b G[a][b] for coefficient vectors in {-1, 0, 1}.
-template <int D>
-__device__ Ext<D> view_bilinear(const Ext<D> (&G)[16], const int (&al)[4], const int (&be)[4]) {
-  Ext<D> r = gl::ext_zero<D>();
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int c = al[a] * be[b];
-      if (c > 0) r = gl::eadd<D>(r, G[a * 4 + b]);
-      else if (c < 0) r = gl::esub<D>(r, G[a * 4 + b]);
-    }
-  return r;
-}
-
 // Sums the blocks' G contributions and writes the pass's NS sums, the same values the
 // pairs kernel's accumulators hold (see k_sc_pair: quarters 0..3 of x and y, dx0 = x2 - x0,
 // dx1 = x3 - x1, ux = x1 - x0, vx = dx1 - dx0, likewise for y).
@@ -712,24 +751,44 @@ __global__ __launch_bounds__(RT) void k_view_reduce(const uint64_t* parts, size_
     for (size_t h = threadIdx.x; h < Qb; h += blockDim.x)
 #pragma unroll
       for (int b = 0; b < 4; b++) G[a * 4 + b] = gl::eadd<D>(G[a * 4 + b], ld<D>(parts, (a * Qb + h) * 4 + b));
-  block_sum<D, 16>(G);
-  if (threadIdx.x != 0) return;
-  const int e0[4] = {1, 0, 0, 0}, e1[4] = {0, 1, 0, 0}, d0[4] = {-1, 0, 1, 0}, d1[4] = {0, -1, 0, 1},
-            u[4] = {-1, 1, 0, 0}, w[4] = {1, -1, -1, 1};
-  Ext<D> acc[8];
-  acc[0] = gl::eadd<D>(view_bilinear<D>(G, e0, e0), view_bilinear<D>(G, e1, e1));  // x0 y0 + x1 y1
-  acc[1] = gl::eadd<D>(view_bilinear<D>(G, d0, d0), view_bilinear<D>(G, d1, d1));  // dx0 dy0 + dx1 dy1
-  if constexpr (NS == 8) {
-    acc[2] = view_bilinear<D>(G, e0, e0);                                                // x0 y0
-    acc[3] = gl::eadd<D>(view_bilinear<D>(G, e0, d0), view_bilinear<D>(G, d0, e0));  // x0 dy0 + dx0 y0
-    acc[4] = view_bilinear<D>(G, d0, d0);                                                // dx0 dy0
-    acc[5] = view_bilinear<D>(G, u, u);                                                  // ux uy
-    acc[6] = gl::eadd<D>(view_bilinear<D>(G, u, w), view_bilinear<D>(G, w, u));        // ux vy + vx uy
-    acc[7] = view_bilinear<D>(G, w, w);                                                  // vx vy
-  }
+  Ext<D> T[16];
+  block_sum_lds<D, 16>(G, T);
+  // Thread k < NS forms sum k = B(al1, be1) + B(al2, be2) from the totals, all threads on
+  // one code path: the coefficient of G[a][b] is al1_a be1_b + al2_a be2_b (in -2..2).
+  __shared__ uint64_t res[8 * D];
+  if (threadIdx.x < NS) {
+    // al1, be1, al2, be2 per sum (coefficient vectors over the quarters 0..3)
+    constexpr int V[8][4][4] = {
+        {{1, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {0, 1, 0, 0}},          // x0 y0 + x1 y1
+        {{-1, 0, 1, 0}, {-1, 0, 1, 0}, {0, -1, 0, 1}, {0, -1, 0, 1}},      // dx0 dy0 + dx1 dy1
+        {{1, 0, 0, 0}, {1, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}},          // x0 y0
+        {{1, 0, 0, 0}, {-1, 0, 1, 0}, {-1, 0, 1, 0}, {1, 0, 0, 0}},        // x0 dy0 + dx0 y0
+        {{-1, 0, 1, 0}, {-1, 0, 1, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}},        // dx0 dy0
+        {{-1, 1, 0, 0}, {-1, 1, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}},        // ux uy
+        {{-1, 1, 0, 0}, {1, -1, -1, 1}, {1, -1, -1, 1}, {-1, 1, 0, 0}},    // ux vy + vx uy
+        {{1, -1, -1, 1}, {1, -1, -1, 1}, {0, 0, 0, 0}, {0, 0, 0, 0}}};     // vx vy
+    const int k = threadIdx.x;
+    Ext<D> r = gl::ext_zero<D>();
 #pragma unroll
-  for (int k = 0; k < NS; k++) st<D>(out, k, acc[k]);
-  mailbox_raise(sig);
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int c = V[k][0][a] * V[k][1][b] + V[k][2][a] * V[k][3][b];
+        const Ext<D>& t = T[a * 4 + b];
+        const Ext<D> z = gl::ext_zero<D>();
+        r = gl::eadd<D>(r, c > 0 ? t : z);
+        r = gl::esub<D>(r, c < 0 ? t : z);
+        r = gl::eadd<D>(r, c > 1 ? t : z);
+        r = gl::esub<D>(r, c < -1 ? t : z);
+      }
+#pragma unroll
+    for (int d = 0; d < D; d++) res[k * D + d] = r.c[d];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NS * D; k++) out[k] = res[k];
+    mailbox_raise(sig);
+  }
 }
 
 template <int Ds, int D>
