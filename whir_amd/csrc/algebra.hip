@@ -7,6 +7,7 @@
 #include "common.h"
 #include "gl.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -168,7 +169,8 @@ static void finish_reduce(const uint64_t* parts, unsigned nparts, uint64_t* out,
 static unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(blocks_for(n, RT), MAX_BLOCKS); }
 
 // Up to this many element pairs a reducing pass runs as one block (four pairs per
-// lane), which stores its sums directly: cheaper than a second launch.
+// lane), which stores its sums directly: cheaper than a second launch (256 and 64
+// measured no faster at d = 24 on MI355X).
 static constexpr size_t SMALL_PAIRS = 1024;
 
 // Pass a small host constant (D words) to kernels by value.
@@ -1316,6 +1318,44 @@ __global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, const u
   block_reduce_store<D, UNI_MP>(acc, parts + (size_t)blockIdx.x * UNI_MP * D);
 }
 
+// Vectors of at most UNI_SMALL_BLOCKS blocks skip the table launch: each thread forms
+// x_j^(i0) (its first index) by square-and-multiply from the squares x_j^(2^k) passed
+// with the launch, then steps by x_j^RT; one block stores its sums itself.
+static constexpr size_t UNI_SMALL_BLOCKS = 256;
+
+template <int Ds, int D>
+__global__ __launch_bounds__(RT) void k_uni_small(const uint64_t* c, size_t n, UniPow pw, int m, uint64_t* parts,
+                                                  MailboxSig sig) {
+  static_assert(RT == 256, "the step x^RT is the square x^(2^8)");
+  const size_t i0 = blockIdx.x * (size_t)RT * UNI_K + threadIdx.x;
+  Ext<D> acc[UNI_MP];
+#pragma unroll
+  for (int j = 0; j < UNI_MP; j++) {
+    acc[j] = gl::ext_zero<D>();
+    if (j < m && i0 < n) {
+      Ext<D> p = gl::ext_base<D>(1);
+      for (int bit = 0; (i0 >> bit) != 0; bit++)
+        if ((i0 >> bit) & 1) {
+          Ext<D> sq;
+#pragma unroll
+          for (int d = 0; d < D; d++) sq.c[d] = pw.w[(j * UNI_PW + bit) * D + d];
+          p = gl::emul<D>(p, sq);
+        }
+      Ext<D> step;
+#pragma unroll
+      for (int d = 0; d < D; d++) step.c[d] = pw.w[(j * UNI_PW + 8) * D + d];
+      for (int k = 0; k < UNI_K; k++) {
+        const size_t i = i0 + (size_t)k * RT;
+        if (i >= n) break;
+        if constexpr (Ds == 1) acc[j] = gl::eadd<D>(acc[j], gl::escale<D>(p, c[i]));
+        else acc[j] = gl::eadd<D>(acc[j], gl::emul<D>(p, ld<D>(c, i)));
+        p = gl::emul<D>(p, step);
+      }
+    }
+  }
+  block_reduce_store<D, UNI_MP>(acc, parts + (size_t)blockIdx.x * UNI_MP * D, sig);
+}
+
 template <int Ds, int D>
 static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uint64_t* out, hipStream_t s,
                   bool padded) {
@@ -1335,7 +1375,9 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     L.B = L.P + mj * L.S * D;
     L.words = L.B + mj * L.nb * D;
     L.entries = mj * (L.S + L.nb);
-    DevBuf buf(L.words * 8, s), parts(L.nb * UNI_MP * D * 8, s), res(UNI_MP * D * 8, s);
+    const bool small = L.nb <= UNI_SMALL_BLOCKS;
+    DevBuf buf(small ? 0 : L.words * 8, s), parts(L.nb > 1 ? L.nb * UNI_MP * D * 8 : 0, s),
+        res((mj == UNI_MP || padded) ? 0 : UNI_MP * D * 8, s);
     WHIR_REQUIRE((uint64_t)(L.nb - 1) * L.S < ((uint64_t)1 << UNI_PW), "univariate: vector too long");
     UniPow pw;
     for (size_t j = 0; j < mj; j++) {
@@ -1346,17 +1388,29 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
         p = gl::emul<D>(p, p);
       }
     }
-    k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L, pw);
-    WHIR_HIP(hipGetLastError());
     // The reduction writes all UNI_MP slots; only the first mj go to the caller
     // (straight into `out` when it has room for whole groups).
     uint64_t* dst = (mj == UNI_MP || padded) ? o : res.as<uint64_t>();
+    const bool last = j0 + UNI_MP >= m;
+    if (small) {
+      {
+        ProfScope prof(kernel_name("k_uni_small", Ds, D), s, 8.0 * Ds * (double)n);
+        k_uni_small<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, pw, (int)mj, L.nb == 1 ? dst : parts.as<uint64_t>(),
+                                                       L.nb == 1 && last ? mailbox_take(dst) : MailboxSig{});
+        WHIR_HIP(hipGetLastError());
+      }
+      if (L.nb > 1) finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s, last);
+      if (dst != o) WHIR_HIP(hipMemcpyAsync(o, dst, mj * D * 8, hipMemcpyDeviceToDevice, s));
+      continue;
+    }
+    k_uni_tables<D><<<blocks_for(L.entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L, pw);
+    WHIR_HIP(hipGetLastError());
     {
       ProfScope prof(kernel_name("k_uni", Ds, D), s, 8.0 * Ds * (double)n);
       k_uni<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, buf.as<uint64_t>(), L, (int)mj, parts.as<uint64_t>());
       WHIR_HIP(hipGetLastError());
     }
-    finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s, j0 + UNI_MP >= m);
+    finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s, last);
     if (dst != o) WHIR_HIP(hipMemcpyAsync(o, dst, mj * D * 8, hipMemcpyDeviceToDevice, s));
   }
 }
