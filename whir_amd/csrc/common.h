@@ -56,14 +56,16 @@ void download_sync(void* dst, const void* src, size_t bytes, hipStream_t s);
 // host memory, so device stores to them are not held in the GPU's L2.
 constexpr size_t RESULT_MAILBOX_WORDS = 65536;
 uint64_t* result_mailbox(hipStream_t s);
-// Waits until everything queued on s so far has finished, for a host that is about to
-// read the mailbox: a new sequence number is stored to a coherent flag word after the
-// mailbox with a system-scope release, and the host spins on the flag (about 6 us from
-// kernel end to host, against about 12 us for hipStreamSynchronize on MI355X).  The
-// store comes from the kernel that wrote the awaited result when the wait was armed
-// (mailbox_arm) and that kernel took the request (mailbox_take), else from a
-// one-thread kernel queued behind the stream's work.  A stream error ends the wait
-// with that error.  Not for use while s is being captured.
+// Waits until the results the host is about to read from the mailbox have landed: a
+// new sequence number is stored to a coherent flag word after the mailbox with a
+// system-scope release, and the host spins on the flag (about 6 us from kernel end to
+// host, against about 12 us for hipStreamSynchronize on MI355X).  The store comes from
+// the kernel that wrote the awaited result when the wait was armed (mailbox_arm) and
+// that kernel took the request (mailbox_take): everything queued before that launch
+// has then finished, and work queued after it (none of which the host reads) may still
+// run.  Otherwise a one-thread kernel queued behind the stream's work stores it, and
+// everything queued so far has finished.  A stream error ends the wait with that
+// error.  Not for use while s is being captured.
 void mailbox_wait(hipStream_t s);
 
 // A request that a kernel storing a result into the mailbox also raise the flag.
@@ -78,7 +80,7 @@ struct MailboxSig {
 void mailbox_arm(hipStream_t s);
 // The armed request when `out` (the destination of the launch about to be made) lies
 // in the armed stream's mailbox, which disarms it; otherwise an empty request.  Only a
-// launch that is the last of the awaited work may take it.
+// launch after which nothing the host will read is still to be written may take it.
 MailboxSig mailbox_take(const void* out);
 #if defined(__HIP__)
 // Device side, by the one thread that stored the result (after its stores).

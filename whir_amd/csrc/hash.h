@@ -24,6 +24,11 @@ void hash_rows_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t 
 // The root also goes to root_out (32 bytes, device or mapped host memory) when given.
 void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
                      uint8_t* root_out = nullptr);
+// commit_cols_dev of a codeword whose N rows are all equal (N a power of two), such as
+// the codeword of messages of length one: the same nodes, made from row 0 alone (each
+// level's node is the hash of the level below paired with itself).
+void commit_const_rows_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes,
+                           hipStream_t st, uint8_t* root_out = nullptr);
 
 // commit_cols_dev of `count` codewords of the same shape in shared launches: codeword
 // i at cw_tab[i], its nodes at nodes_tab[i] (device arrays of pointers), its root to

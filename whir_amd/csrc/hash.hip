@@ -626,6 +626,87 @@ void commit_cols_many_dev(int engine, int D, const uint64_t* const* cw_tab, uint
   merkle_from(engine, nullptr, N, N / 4, st, nullptr, tab, count);
 }
 
+// ------------------------------------------------------------------ constant rows
+// Every Merkle level of a codeword with equal rows holds one node value: one lane hashes
+// row 0 and then pairs each level's node with itself up to the root (dig[l] = the node
+// of level l, leaves at l = 0); a fill launch writes them over the levels.
+template <int D, int ENG>
+__global__ void k_const_chain(const uint64_t* __restrict__ cw, size_t N, size_t M, int levels, uint32_t* dig,
+                              uint32_t* root_out, MailboxSig sig) {
+  struct Row0 {
+    const uint64_t* cw;
+    size_t N, size;
+    __device__ uint64_t w64(size_t off) const {
+      if (off >= size) return 0;
+      const size_t word = off >> 3, j = word / D, d = word % D;
+      return cw[j * N * D + d];
+    }
+    __device__ uint32_t w32(size_t off) const { return (uint32_t)(w64(off & ~(size_t)7) >> (8 * (off & 7))); }
+  };
+  if (threadIdx.x != 0) return;
+  uint32_t h[8];
+  hd::hash_msg_t<ENG>(Row0{cw, N, 8 * M * D}, 8 * M * D, h);
+#pragma unroll
+  for (int i = 0; i < 8; i++) dig[i] = h[i];
+  for (int l = 1; l <= levels; l++) {
+    uint32_t o[8];
+    hash_pair<ENG>(h, h, o);
+#pragma unroll
+    for (int i = 0; i < 8; i++) dig[8 * l + i] = h[i] = o[i];
+  }
+  if (root_out) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) root_out[i] = h[i];
+    mailbox_raise(sig);
+  }
+}
+
+// Node e of a tree over W leaves (W a power of two; level widths W, W/2, ..., 1 laid
+// out one after another) gets its level's digest.
+__global__ void k_const_fill(uint8_t* __restrict__ nodes, size_t W, const uint32_t* __restrict__ dig) {
+  const size_t total = 2 * W - 1;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    size_t base = 0, width = W;  // level l covers [base, base + width)
+    int l = 0;
+    while (e >= base + width) {
+      base += width;
+      width >>= 1;
+      l++;
+    }
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) h[i] = dig[8 * l + i];
+    store_digest(nodes + 32 * e, h);
+  }
+}
+
+void commit_const_rows_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes,
+                           hipStream_t st, uint8_t* root_out) {
+  WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D) && engine_supports_size(engine, 64),
+               "commit: engine does not support the row or node size");
+  WHIR_REQUIRE(N > 0 && (N & (N - 1)) == 0, "commit: constant rows need a power-of-two codeword length");
+  const int levels = (int)merkle_layers(N);
+  DevBuf dig((size_t)(levels + 1) * 32, st);
+  {
+    ProfScope prof("k_const_chain", st, 8.0 * M * D);
+    const MailboxSig sig = root_out ? mailbox_take(root_out) : MailboxSig{};
+    engine_switch(engine, [&](auto E) {
+      constexpr int ENG = decltype(E)::value;
+      switch (D) {
+        case 1: k_const_chain<1, ENG><<<1, 64, 0, st>>>(cw, N, M, levels, dig.as<uint32_t>(), (uint32_t*)root_out, sig); break;
+        case 2: k_const_chain<2, ENG><<<1, 64, 0, st>>>(cw, N, M, levels, dig.as<uint32_t>(), (uint32_t*)root_out, sig); break;
+        case 3: k_const_chain<3, ENG><<<1, 64, 0, st>>>(cw, N, M, levels, dig.as<uint32_t>(), (uint32_t*)root_out, sig); break;
+        default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
+      }
+    });
+    WHIR_HIP(hipGetLastError());
+  }
+  ProfScope prof("k_const_fill", st, 32.0 * (2.0 * N - 1));
+  k_const_fill<<<(unsigned)std::min<size_t>(blocks_for(2 * N - 1, 256), 4096), 256, 0, st>>>(nodes, N,
+                                                                                             dig.as<uint32_t>());
+  WHIR_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ PoW
 static constexpr int POW_THREADS = 256;
 static constexpr int POW_PER_LANE = 16;

@@ -239,7 +239,10 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   // The root lands in the stream's pinned mailbox from the last tree launch.
   uint64_t* mbox = result_mailbox(s);
   mailbox_arm(s);
-  commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
+  if (L == 1 && (N & (N - 1)) == 0)  // every column is a constant: all rows are equal
+    commit_const_rows_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
+  else
+    commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
   mailbox_wait(s);
   uint8_t root[32];
   memcpy(root, mbox, 32);
