@@ -52,6 +52,7 @@ def parse():
                     "Goldilocks3 prover timings")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pow", action="store_true", help="skip the PoW grinding-rate measurement")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host-memory (PCIe-inclusive) proof timing")
     ap.add_argument("--workers", type=int, default=8, help="executor worker threads (streams) for serving")
     ap.add_argument("--batch-workers", type=int, default=12,
                     help="executor worker threads for the batched measurement (12 measured ~10%% faster than "
@@ -219,7 +220,7 @@ def main():
     #     time on a copy stream, each chunk's encode overlapping the later copies;
     # (b) for reference, a plain upload from pageable memory before each proof.
     pcie = None
-    if not covs:
+    if not covs and not args.no_pcie:
         pinned = W.PinnedArray(vec.size)
         pinned.array[:] = vec
         dhost = W.DeviceBuffer(vec.nbytes)
