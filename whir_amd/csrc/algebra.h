@@ -97,6 +97,14 @@ void geometric_accumulate_dev(int D, uint64_t* acc, size_t n, const uint64_t* sc
 void covector_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_t* eq_points,
                   const uint64_t* eq_scalars, size_t np, size_t nv, const uint64_t* scalars, const uint64_t* points,
                   size_t m, hipStream_t st);
+// covector_dev's power terms for a round's STIR constraints: m_e points given as field
+// elements (the OOD samples, D words each), then m_b in-domain points x_j = w_N^exps[j]
+// (points[j] holds the same values as base-field words), N = 2^logN; scalars (m_e + m_b
+// elements, D words each) in that order.  Many terms over a long covector take a kernel
+// factored by i = 256 h + l (per-element work: D 64x64 products per in-domain term).
+void covector_stir_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_t* scalars,
+                       const uint64_t* ext_points, size_t m_e, const uint64_t* exps, const uint64_t* points, size_t m_b,
+                       int logN, hipStream_t st);
 
 // mixed_univariate_evaluate at m host points x[0..m) in shared passes; results to
 // dev_out[0..m D).

@@ -6,6 +6,7 @@
 #include "algebra.h"
 #include "common.h"
 #include "gl.h"
+#include "ntt.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -1004,6 +1005,150 @@ static int cov_k(size_t n, int D) {
   return K;
 }
 
+// ------------------------------------------------------------------ many power terms
+// sum_j t_j x_j^i for many points (the STIR terms of a round: the OOD samples and the
+// in-domain queries), factored by i = 256 h + l:
+//   t_j x_j^i = HI_j[h] * LO_j[l],  HI_j[h] = t_j x_j^(256 h),  LO_j[l] = x_j^l.
+// Lane l of a block holds LO_j[l] for one point at a time and adds it times the block's
+// K high values (the same for every lane: scalar loads) into 128-bit sums, reduced once
+// per element at the end; in-domain points lie in the base field, so their terms cost D
+// 64x64 products per element instead of an extension scale and an addition.  In-domain
+// points come as exponents of the domain's root of unity (x = w_N^e), so each of their
+// table entries is one twiddle-table product (exponents multiply mod N); the few OOD
+// points' tables are powers computed in short runs.
+constexpr int GEO_K = 4;        // rows h per block
+constexpr int GEO_RUN = 8;      // consecutive OOD-point table entries per thread
+struct GeoPlan {
+  size_t m_e, m_b, H;           // OOD points first, then in-domain points; H = n / 256
+  size_t in_x, in_t, in_e;      // inputs: OOD points (m_e D words), scalars (all points),
+                                // in-domain exponents (m_b words)
+  size_t hi, lo_e, lo_b;        // tables: HI (m H elements), LO of OOD points (m_e 256
+                                // elements), LO of in-domain points (m_b 256 words)
+  uint64_t N;                   // domain size of the in-domain points (a power of two)
+};
+
+template <int D>
+__global__ void k_geo_tables(uint64_t* buf, GeoPlan L, const uint64_t* in_dev, CovIn in_arg, TwTable tw) {
+  const size_t runs_hi = (L.H + GEO_RUN - 1) / GEO_RUN;
+  const size_t sec[4] = {L.m_b * L.H, L.m_b * 256, L.m_e * runs_hi, L.m_e * (256 / GEO_RUN)};
+  size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e < sec[0]) {  // in-domain HI: t_j w^(256 h e_j)
+    const size_t j = e / L.H, h = e % L.H;
+    const uint64_t ej = in_dev ? in_dev[L.in_e + j] : in_arg.w[L.in_e + j];
+    const uint64_t x = tw_at(tw, ((uint64_t)h * 256 * ej) & (L.N - 1));
+    st<D>(buf + L.hi, (L.m_e + j) * L.H + h, gl::escale<D>(in_ld<D>(in_dev, in_arg, L.in_t, L.m_e + j), x));
+    return;
+  }
+  e -= sec[0];
+  if (e < sec[1]) {  // in-domain LO: w^(l e_j)
+    const size_t j = e / 256, l = e % 256;
+    const uint64_t ej = in_dev ? in_dev[L.in_e + j] : in_arg.w[L.in_e + j];
+    buf[L.lo_b + e] = tw_at(tw, ((uint64_t)l * ej) & (L.N - 1));
+    return;
+  }
+  e -= sec[1];
+  if (e < sec[2]) {  // OOD HI in runs: t_j x_j^(256 h)
+    const size_t j = e / runs_hi, h0 = (e % runs_hi) * GEO_RUN;
+    const Ext<D> x = in_ld<D>(in_dev, in_arg, L.in_x, j);
+    const Ext<D> step = gl::epow<D>(x, 256);
+    Ext<D> v = gl::emul<D>(in_ld<D>(in_dev, in_arg, L.in_t, j), gl::epow<D>(x, (uint64_t)h0 * 256));
+    for (size_t h = h0; h < h0 + GEO_RUN && h < L.H; h++) {
+      st<D>(buf + L.hi, j * L.H + h, v);
+      v = gl::emul<D>(v, step);
+    }
+    return;
+  }
+  e -= sec[2];
+  if (e >= sec[3]) return;
+  const size_t j = e / (256 / GEO_RUN), l0 = (e % (256 / GEO_RUN)) * GEO_RUN;  // OOD LO in runs: x_j^l
+  const Ext<D> x = in_ld<D>(in_dev, in_arg, L.in_x, j);
+  Ext<D> v = gl::epow<D>(x, l0);
+  for (size_t l = l0; l < l0 + GEO_RUN; l++) {
+    st<D>(buf + L.lo_e, j * 256 + l, v);
+    v = gl::emul<D>(v, x);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_geo(uint64_t* acc, size_t n, int accumulate, const uint64_t* buf, GeoPlan L) {
+  const size_t l = threadIdx.x, h0 = (size_t)blockIdx.x * GEO_K;
+  gl::Wide W[GEO_K][D];
+#pragma unroll
+  for (int k = 0; k < GEO_K; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) W[k][d] = gl::Wide{0, 0, 0};
+  const uint64_t* hi = buf + L.hi;
+  // in-domain points: HI (extension) x LO (base), D products per row
+  for (size_t j = 0; j < L.m_b; j++) {
+    const uint64_t lo = buf[L.lo_b + j * 256 + l];
+    const uint64_t* hj = hi + ((L.m_e + j) * L.H + h0) * D;
+#pragma unroll
+    for (int k = 0; k < GEO_K; k++)
+#pragma unroll
+      for (int d = 0; d < D; d++) gl::wide_mac(W[k][d], hj[k * D + d], lo);
+  }
+  Ext<D> sum[GEO_K];
+#pragma unroll
+  for (int k = 0; k < GEO_K; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) sum[k].c[d] = gl::wide_reduce(W[k][d]);
+  // OOD points: full extension products
+  for (size_t j = 0; j < L.m_e; j++) {
+    const Ext<D> lo = ld<D>(buf + L.lo_e, j * 256 + l);
+#pragma unroll
+    for (int k = 0; k < GEO_K; k++) sum[k] = gl::eadd<D>(sum[k], gl::emul<D>(ld<D>(hi, j * L.H + h0 + k), lo));
+  }
+#pragma unroll
+  for (int k = 0; k < GEO_K; k++) {
+    const size_t i = (h0 + k) * 256 + l;
+    if (i < n) st<D>(acc, i, accumulate ? gl::eadd<D>(ld<D>(acc, i), sum[k]) : sum[k]);
+  }
+}
+
+template <int D>
+static void geo_terms_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t* scalars, const uint64_t* ext_points,
+                        size_t m_e, const uint64_t* exps, size_t m_b, int logN, hipStream_t s) {
+  GeoPlan L{};
+  L.H = n / 256;
+  L.m_e = m_e;
+  L.m_b = m_b;
+  L.N = (uint64_t)1 << logN;
+  const size_t m = m_e + m_b;
+  std::vector<uint64_t> in;
+  L.in_x = 0;
+  in.insert(in.end(), ext_points, ext_points + m_e * D);
+  L.in_t = in.size();
+  in.insert(in.end(), scalars, scalars + m * D);
+  L.in_e = in.size();
+  in.insert(in.end(), exps, exps + m_b);
+  const bool by_arg = in.size() <= COV_ARG_WORDS;
+  size_t w = by_arg ? 0 : in.size();
+  L.hi = w;   w += m * L.H * D;
+  L.lo_e = w; w += m_e * 256 * D;
+  L.lo_b = w; w += m_b * 256;
+  DevBuf buf(w * 8, s);
+  CovIn args{};
+  if (by_arg) std::copy(in.begin(), in.end(), args.w);
+  else upload_async(buf.ptr, in.data(), in.size() * 8, s);
+  const TwTable tw = m_b ? twiddles(logN) : TwTable{};
+  const size_t entries = m_b * L.H + m_b * 256 + m_e * ((L.H + GEO_RUN - 1) / GEO_RUN) + m_e * (256 / GEO_RUN);
+  k_geo_tables<D><<<blocks_for(entries, RT), RT, 0, s>>>(buf.as<uint64_t>(), L, by_arg ? nullptr : buf.as<uint64_t>(),
+                                                          args, tw);
+  WHIR_HIP(hipGetLastError());
+  ProfScope prof(kernel_name("k_geo", D), s, 8.0 * D * (accumulate ? 2.0 : 1.0) * (double)n);
+  k_geo<D><<<(unsigned)(L.H / GEO_K), 256, 0, s>>>(acc, n, accumulate ? 1 : 0, buf.as<uint64_t>(), L);
+  WHIR_HIP(hipGetLastError());
+}
+
+// WHIR_COV_GEO=0 keeps power terms on the one-pass kernel (for comparison).
+static bool geo_path_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("WHIR_COV_GEO");
+    return !e || atoi(e) != 0;
+  }();
+  return on;
+}
+
 template <int D>
 static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t* eq_points,
                        const uint64_t* eq_scalars, size_t np, size_t nv, const uint64_t* scalars,
@@ -1235,6 +1380,28 @@ void covview_write_dev(int D, const CovView& v, uint64_t* out, size_t n, hipStre
     default: throw Error("covector view: extension degree must be 1, 2 or 3");
   }
   WHIR_HIP(hipGetLastError());
+}
+
+void covector_stir_dev(int D, uint64_t* acc, size_t n, bool accumulate, const uint64_t* scalars,
+                       const uint64_t* ext_points, size_t m_e, const uint64_t* exps, const uint64_t* points, size_t m_b,
+                       int logN, hipStream_t s) {
+  const size_t m = m_e + m_b;
+  if (!geo_path_enabled() || m < 8 || n < ((size_t)1 << 14) || n % (256 * GEO_K) || logN > 62) {
+    // the one-pass kernel, with every point as a field element (in-domain points lifted)
+    std::vector<uint64_t> pts(ext_points, ext_points + m_e * D);
+    for (size_t j = 0; j < m_b; j++) {
+      pts.push_back(points[j]);
+      for (int d = 1; d < D; d++) pts.push_back(0);
+    }
+    covector_dev(D, acc, n, accumulate, nullptr, nullptr, 0, 0, scalars, pts.data(), m, s);
+    return;
+  }
+  switch (D) {
+    case 1: geo_terms_t<1>(acc, n, accumulate, scalars, ext_points, m_e, exps, m_b, logN, s); break;
+    case 2: geo_terms_t<2>(acc, n, accumulate, scalars, ext_points, m_e, exps, m_b, logN, s); break;
+    case 3: geo_terms_t<3>(acc, n, accumulate, scalars, ext_points, m_e, exps, m_b, logN, s); break;
+    default: throw Error("covector: extension degree must be 1, 2 or 3");
+  }
 }
 
 void eval_eq_dev(int D, uint64_t* acc, const uint64_t* point, size_t k, const uint64_t* scalar, hipStream_t s) {

@@ -6,6 +6,8 @@
 
 #include <string>
 
+#include "gl.h"
+
 namespace whir {
 
 // One interleaved encode: M messages, message j at msgs + j*msg_stride*D
@@ -31,6 +33,22 @@ struct RsArgs {
   uint64_t* const* out_tab = nullptr;
   uint64_t mpi = 1;
 };
+
+// Powers of w_n, the order-n root of unity of the reference's NTT (n = 2^logn), as two
+// device tables: w_n^e = lo[e & (2^h - 1)] * hi[e >> h] for e < n.  Built on first use
+// per (device, logn), kept for the life of the process; not during a stream capture.
+struct TwTable {
+  const uint64_t* lo = nullptr;  // w_n^j, j < 2^h
+  const uint64_t* hi = nullptr;  // w_n^(j 2^h)
+  int h = 0;
+};
+TwTable twiddles(int logn);
+#if defined(__HIP__)
+__device__ __forceinline__ uint64_t tw_at(const TwTable& t, uint64_t e) {
+  const uint64_t mask = (1ull << t.h) - 1;
+  return gl::mul(t.lo[e & mask], t.hi[e >> t.h]);
+}
+#endif
 
 // Words of device scratch rs_encode_cols needs for these sizes.
 size_t rs_scratch_words(int D, uint64_t M, uint64_t N, uint64_t C);

@@ -79,11 +79,6 @@ static void allow_lds(K kernel, size_t bytes) {
 // --------------------------------------------------------------------------
 // Twiddle tables.
 // --------------------------------------------------------------------------
-struct TwTable {
-  const uint64_t* lo = nullptr;  // w_n^j, j < 2^h
-  const uint64_t* hi = nullptr;  // w_n^(j 2^h)
-  int h = 0;
-};
 
 namespace {
 std::mutex g_tw_mu;
@@ -144,10 +139,6 @@ static const uint64_t* flat_twiddles(int logS) {
   return p;
 }
 
-__device__ __forceinline__ uint64_t tw_at(const TwTable& t, uint64_t e) {
-  uint64_t mask = (1ull << t.h) - 1;
-  return gl::mul(t.lo[e & mask], t.hi[e >> t.h]);
-}
 
 // kind 0: pre[c*C + i] = w_N^(c i)                        (c < Q, i < C)
 // kind 1: post[(c*C2 + t2)*C1 + i1] = w_N^(i1 (c + Q t2))  (pass-1 output order)

@@ -272,7 +272,9 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
 }
 
 // in_domain_challenges (irs_commit.rs:486-500): indices and their evaluation points.
-void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::vector<uint64_t>& pts) {
+// exps (optional) receives each point's exponent e (the point is g^e, g of order N).
+void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::vector<uint64_t>& pts,
+               std::vector<uint64_t>* exps = nullptr) {
   idx = challenge_indices(t, c.codeword_length, c.in_domain_samples, c.deduplicate_in_domain);
   const size_t N = c.codeword_length;
   uint64_t C = coset_size(c.masked_message_length(), N), Q = N / C;
@@ -294,8 +296,10 @@ void in_domain(Transcript& t, const IrsCfg& c, std::vector<size_t>& idx, std::ve
     base = gl::mul(row[15], base);
   }
   pts.clear();
+  if (exps) exps->clear();
   for (size_t i : idx) {
     const uint64_t e = (i / C) + (i % C) * Q;
+    if (exps) exps->push_back(e);
     uint64_t v = 1;
     for (int w = 0; w < windows; w++) {
       const unsigned d = (unsigned)(e >> (4 * w)) & 15;
@@ -415,9 +419,10 @@ constexpr size_t OPEN_MAILBOX_OFF = 64;
 // draws the indices, computes their points and queues every witness's gather.
 std::vector<OpenLaunch> irs_open_launch(Transcript& t, const IrsCfg& c, const std::vector<const DevWitness*>& ws,
                                         std::vector<uint64_t>& pts, hipStream_t s, int D = 0,
-                                        const std::vector<std::vector<uint64_t>>* wts = nullptr) {
+                                        const std::vector<std::vector<uint64_t>>* wts = nullptr,
+                                        std::vector<uint64_t>* exps = nullptr) {
   std::vector<size_t> idx;
-  in_domain(t, c, idx, pts);
+  in_domain(t, c, idx, pts, exps);
   std::vector<OpenLaunch> ls;
   size_t off = OPEN_MAILBOX_OFF;
   for (size_t w = 0; w < ws.size(); w++) {
@@ -1099,19 +1104,17 @@ void prove_impl(const WhirCfg& cfg, Transcript& t, const uint64_t* const* vecs, 
     // back to back with one synchronization: the STIR challenge depends on the
     // transcript alone (hints do not enter it), and the in-domain values, which only
     // the claimed sum needs, come back with the first pass's sums.
-    std::vector<uint64_t> pts;
+    std::vector<uint64_t> pts, exps;
     const std::vector<std::vector<uint64_t>> wts = column_weights(prev_ws, vector_rlc, eq_weights<D>(fr));
-    std::vector<OpenLaunch> opens = irs_open_launch(t, *prev_cfg, prev_ws, pts, s, D, &wts);
+    std::vector<OpenLaunch> opens = irs_open_launch(t, *prev_cfg, prev_ws, pts, s, D, &wts, &exps);
     // STIR constraints: OOD points of the new commitment, then the opened rows.
     const size_t nood = nw->ood_values.size() / D, kq = pts.size();
-    std::vector<uint64_t> spoints(nw->ood_points);
-    for (uint64_t x : pts) push<D>(spoints, gl::ext_base<D>(x));
     std::vector<E<D>> srlc = geometric_challenge<D>(t, nood + kq);
     if (!srlc.empty()) {
       std::vector<uint64_t> flat;
       for (auto& e : srlc) push<D>(flat, e);
-      covector_dev(D, cov.as<uint64_t>(), n, true, nullptr, nullptr, 0, 0, flat.data(), spoints.data(),
-                   srlc.size(), s);
+      covector_stir_dev(D, cov.as<uint64_t>(), n, true, flat.data(), nw->ood_points.data(), nood, exps.data(),
+                        pts.data(), kq, log2_exact(prev_cfg->codeword_length), s);
     }
     for (size_t k = 0; k < nood; k++)
       the_sum = gl::eadd<D>(the_sum, gl::emul<D>(srlc[k], ld<D>(&nw->ood_values[k * D])));
