@@ -132,6 +132,9 @@ int whir_ntt_schedules(char *buf, size_t cap, size_t *len);
 int whir_hash_engine_id(int engine, uint8_t out[32]);
 /* supports_size (src/hash/mod.rs:65-66).  *out = 1 if supported. */
 int whir_hash_supports_size(int engine, size_t size, int *out);
+/* preferred_batch_size (src/hash/mod.rs:67-77): messages per whir_hash_many call that
+ * fill the current device in one launch (one message per lane: 1024 per CU). */
+int whir_hash_preferred_batch_size(int engine, size_t *out);
 /* hash_many (src/hash/mod.rs:79-85): `count` messages of `size` bytes, concatenated. */
 int whir_hash_many(int engine, size_t size, const uint8_t *input, size_t count, uint8_t *output);
 int whir_hash_many_dev(int engine, size_t size, const uint8_t *input, size_t count, uint8_t *output);

@@ -101,6 +101,17 @@ def test_hash_many_matches_oracle(engine):
         assert got == O.hash_many(engine, size, data, count), (engine, size)
 
 
+def test_preferred_batch_size():
+    """HashEngine::preferred_batch_size (src/hash/mod.rs:67-77): one message per lane,
+    1024 per CU of the current device; an unknown engine is an error."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    for e in (W.HASH_COPY, W.HASH_SHA2, W.HASH_KECCAK, W.HASH_SHA3, W.HASH_BLAKE3):
+        assert W.Engine(e).preferred_batch_size() == 1024 * cus
+    with pytest.raises(W.WhirError):
+        W.Engine(99).preferred_batch_size()
+
+
 def test_hash_many_copy_engine_and_empty():
     data = bytes(range(24)) * 5
     assert W.Engine(W.HASH_COPY).hash_many(24, data) == O.hash_many(0, 24, data, 5)
@@ -117,13 +128,15 @@ def test_merkle_commit_matches_oracle(engine, n):
     assert W.merkle_commit(engine, leaves) == O.merkle_commit(engine, leaves)
 
 
-@pytest.mark.parametrize("n", [1 << 19, (1 << 20) + 3])
-def test_merkle_commit_wide_levels_match_oracle(n):
-    """Trees wide enough for the eight-nodes-per-lane first phase of the subtree
-    kernel (levels of 2^19 nodes and more), then its pair levels, for BLAKE3."""
+@pytest.mark.parametrize("engine,n", [(W.HASH_BLAKE3, 1 << 19), (W.HASH_BLAKE3, (1 << 20) + 3),
+                                      (W.HASH_SHA3, (1 << 22) + 5)])
+def test_merkle_commit_wide_levels_match_oracle(engine, n):
+    """Trees with levels of 2^19 nodes and more: three-level launches of eight nodes
+    per lane (2^20 -> 2^17, and for the 2^23-wide padded tree 2^23 -> 2^20 -> 2^17),
+    then the block-wide subtree kernel above them."""
     rng = np.random.default_rng(n)
     leaves = rng.integers(0, 256, size=32 * n, dtype=np.uint8).tobytes()
-    assert W.merkle_commit(W.HASH_BLAKE3, leaves) == O.merkle_commit(W.HASH_BLAKE3, leaves)
+    assert W.merkle_commit(engine, leaves) == O.merkle_commit(engine, leaves)
 
 
 @pytest.mark.parametrize("D", [1, 2, 3])

@@ -75,6 +75,7 @@ SIGNATURES = {
     "whir_rs_interleaved_encode_dev": [ci, vp, sz, sz, vp, sz, sz, vp, ci],
     "whir_hash_engine_id": [ci, u8p],
     "whir_hash_supports_size": [ci, sz, ctypes.POINTER(ci)],
+    "whir_hash_preferred_batch_size": [ci, ctypes.POINTER(sz)],
     "whir_hash_many": [ci, sz, u8p, sz, u8p],
     "whir_hash_many_dev": [ci, sz, vp, sz, vp],
     "whir_merkle_num_nodes": [sz, szp],
@@ -274,7 +275,9 @@ class Engine:
         return bool(r.value)
 
     def preferred_batch_size(self) -> int:
-        return 1 << 16
+        r = sz()
+        _call("whir_hash_preferred_batch_size", self.engine, ctypes.byref(r))
+        return r.value
 
     def hash_many(self, size: int, data: bytes) -> bytes:
         if size == 0:

@@ -300,6 +300,12 @@ int whir_hash_supports_size(int engine, size_t size, int* out) {
   API_END
 }
 
+int whir_hash_preferred_batch_size(int engine, size_t* out) {
+  API_BEGIN
+  *out = engine_preferred_batch(engine);
+  API_END
+}
+
 int whir_hash_many(int engine, size_t size, const uint8_t* input, size_t count, uint8_t* output) {
   API_BEGIN
   WHIR_REQUIRE(engine_supports_size(engine, size), "hash_many: engine does not support this message size");

@@ -10,6 +10,7 @@ namespace whir {
 enum Engine : int { ENG_COPY = 0, ENG_SHA2 = 1, ENG_KECCAK = 2, ENG_SHA3 = 3, ENG_BLAKE3 = 4 };
 
 bool engine_supports_size(int engine, size_t size);
+size_t engine_preferred_batch(int engine);
 
 // count messages of `size` bytes each (contiguous) -> count 32-byte digests.
 void hash_many_dev(int engine, size_t size, const uint8_t* in, size_t count, uint8_t* out, hipStream_t st);

@@ -7,9 +7,11 @@
 //                  elements are canonical, so the ArkFieldEncoder bytes (canonical
 //                  little-endian u64 per base coefficient, matrix_commit.rs:130-156) are
 //                  the words as stored.
-// merkle           merkle_tree::Config::commit (src/protocols/merkle_tree.rs:82-127): each
-//                  block reduces a 2^k-node slice of one level through k levels in LDS,
-//                  writing every node it makes to its place in the nodes array.
+// merkle           merkle_tree::Config::commit (src/protocols/merkle_tree.rs:82-127): levels
+//                  of 2^19 nodes and more lose three levels per launch, each lane reducing
+//                  eight nodes in registers; above them each block reduces a 2^k-node
+//                  slice of one level through k levels in LDS.  Every node made is
+//                  written to its place in the nodes array.
 // pow              proof_of_work::Config::prove (src/protocols/proof_of_work.rs:68-166):
 //                  persistent search over nonce chunks handed out in increasing order; the
 //                  result is the globally minimal nonce.
@@ -63,6 +65,19 @@ static void engine_switch(int engine, F&& f) {
     case ENG_BLAKE3: f(std::integral_constant<int, ENG_BLAKE3>{}); break;
     default: throw Error("unknown hash engine");
   }
+}
+
+// HashEngine::preferred_batch_size (src/hash/mod.rs:67-77): the CPU engines give their
+// SIMD width (blake3_engine.rs:79-81) or 1 (digest_engine.rs:118-120, copy_engine.rs:38-40),
+// the messages one thread hashes at once.  hash_many here hashes one message per lane,
+// so the batch that fills the device in one launch is every CU's four SIMDs holding four
+// waves of 64 lanes: 1024 messages per CU (262144 on a 256-CU MI355X).
+size_t engine_preferred_batch(int engine) {
+  engine_switch(engine, [](auto) {});
+  int dev = 0, cus = 0;
+  WHIR_HIP(hipGetDevice(&dev));
+  WHIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return (size_t)std::max(cus, 1) * 1024;
 }
 
 template <int ENG>
@@ -351,6 +366,36 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restri
   }
 }
 
+// Three Merkle levels above a level of width w (w % 2^11 == 0): lane g reduces nodes
+// 8g..8g+7 to one in registers and stores the seven nodes it makes.  No lane idles
+// and nothing waits at a barrier, which matters on levels wide enough to keep every
+// SIMD busy: there the block-wide reduction's later phases (128, 64, ... lanes of a
+// 256-lane block) hold registers that waiting blocks could use.
+template <int ENG>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_wide(uint8_t* __restrict__ nodes, size_t W, size_t w,
+                                                                BatchTab tab) {
+  size_t blk = blockIdx.x;
+  if (tab.nodes) {
+    nodes = tab.nodes[blk / tab.per];
+    blk %= tab.per;
+  }
+  const size_t f = (blk * MERKLE_THREADS + threadIdx.x) << 3;
+  uint32_t h[8][8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint4* src = (const uint4*)(nodes + 32 * (2 * W - 2 * w + f + k));
+    const uint4 a = src[0], b = src[1];
+    h[k][0] = a.x; h[k][1] = a.y; h[k][2] = a.z; h[k][3] = a.w;
+    h[k][4] = b.x; h[k][5] = b.y; h[k][6] = b.z; h[k][7] = b.w;
+  }
+  reduce_levels<3, ENG, 8>(h, nodes, W, w, f);
+}
+
+// Levels at least this wide go through k_merkle_wide (2^16 lanes and more: a wave or
+// more per SIMD).  Measured on MI355X, d = 26 SHA3 proofs: 20.6 -> 19.2 ms with 2^19
+// or 2^18, 19.6 ms with 2^20; BLAKE3 trees, latency-bound at these widths, unchanged.
+static constexpr size_t MERKLE_WIDE_MIN = (size_t)1 << 19;
+
 // Merkle levels above a level of width w (a power of two) already in place.  The
 // root also goes to root_out (32 bytes) when given.  With a batch table, the same
 // levels of `count` trees (root i to tab.roots + 32 i).
@@ -360,6 +405,16 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     WHIR_REQUIRE(!tab.nodes, "merkle: a batch of one-leaf trees is not supported");
     if (root_out) WHIR_HIP(hipMemcpyAsync(root_out, nodes + 32 * (2 * W - 2), 32, hipMemcpyDefault, st));
     return;
+  }
+  while (w >= MERKLE_WIDE_MIN) {
+    const size_t blocks = w / ((size_t)MERKLE_THREADS << 3);
+    ProfScope prof("k_merkle_wide", st, 32.0 * (double)(w + (w >> 1) + (w >> 2) + (w >> 3)) * count);
+    tab.per = (uint32_t)blocks;
+    engine_switch(engine, [&](auto E) {
+      k_merkle_wide<decltype(E)::value><<<(unsigned)(blocks * count), MERKLE_THREADS, 0, st>>>(nodes, W, w, tab);
+    });
+    WHIR_HIP(hipGetLastError());
+    w >>= 3;
   }
   while (w > 1) {
     // Wide levels: eight nodes per lane first (2^11 per block); otherwise pairs
