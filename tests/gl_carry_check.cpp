@@ -1,6 +1,7 @@
 // CPU check of the GPU formulations of Goldilocks arithmetic (whir_amd/csrc/gl.h
 // with GL_HOST_CARRY_TEST: the carry-chain add/sub, the 4-product 128-bit multiply,
-// the carry-chain reduction and mul_pow2) against unsigned __int128 arithmetic,
+// the carry-chain reduction, mul_pow2 and the 128-bit-plus-carries sums of products)
+// against unsigned __int128 arithmetic,
 // over every pair of edge values and a stream of pseudo-random pairs.
 // Prints "ok <count>" or the first mismatch; exit status 0 iff all agree.
 #define GL_HOST_CARRY_TEST 1
@@ -85,6 +86,56 @@ static int pair(uint64_t a, uint64_t b) {
   return bad;
 }
 
+// Sums of products in the 128-bit-plus-carries accumulator (wide_mac, wide_add,
+// wide_sub, wide_reduce) against an exact 192-bit sum (u128 + carry count).
+struct Exact {
+  u128 low;
+  uint64_t top;
+};
+static void ex_add(Exact& e, u128 v) {
+  const u128 s = e.low + v;
+  e.top += s < e.low;
+  e.low = s;
+}
+static int wide_check(const char* op, const gl::Wide& w, const Exact& e) {
+  int bad = 0;
+  bad |= check(op, (uint64_t)e.low, 0, w.lo, (uint64_t)e.low);
+  bad |= check(op, (uint64_t)(e.low >> 64), 1, w.hi, (uint64_t)(e.low >> 64));
+  bad |= check(op, e.top, 2, w.top, e.top);
+  // value mod p = low + top 2^128, 2^128 = -2^32 (mod p)
+  const uint64_t want = r_sub((uint64_t)(e.low % P), r_mul(e.top % P, 1ull << 32));
+  bad |= check(op, e.top, 3, gl::wide_reduce(w), want);
+  return bad;
+}
+static int wide_sums(const uint64_t* vals, int nv, long rounds) {
+  int bad = 0;
+  for (long r = 0; r < rounds && !bad; r++) {
+    gl::Wide w{0, 0, 0};
+    Exact e{0, 0};
+    const int len = 1 + (int)(rnd() % 300);
+    for (int i = 0; i < len && !bad; i++) {
+      const uint64_t a = (rnd() & 3) ? rnd() : vals[rnd() % nv], b = (rnd() & 3) ? rnd() : vals[rnd() % nv];
+      gl::wide_mac(w, a, b);
+      ex_add(e, (u128)a * b);
+      if ((i & 31) == 31) {  // a whole product added and taken away again
+        const gl::Wide x = gl::wide_prod(a, b);
+        gl::wide_add(w, x);
+        gl::wide_sub(w, x);
+      }
+    }
+    bad |= wide_check("wide_mac", w, e);
+  }
+  // every product of two all-ones words: each one carries out of 128 bits in turn
+  gl::Wide w{0, 0, 0};
+  Exact e{0, 0};
+  for (int i = 0; i < 1000 && !bad; i++) {
+    gl::wide_mac(w, ~0ull, ~0ull);
+    ex_add(e, (u128)~0ull * ~0ull);
+    if (i % 97 == 0) bad |= wide_check("wide_mac.ones", w, e);
+  }
+  return bad | wide_check("wide_mac.ones", w, e);
+}
+
 int main(int argc, char** argv) {
   long n_random = argc > 1 ? atol(argv[1]) : 2000000;
   const uint64_t E = 0xFFFFFFFFull;
@@ -127,6 +178,7 @@ int main(int argc, char** argv) {
     bad |= pair(a, b);
     if ((t & 63) == 0) bad |= check("mul_pow2", a, t % 192, gl::mul_pow2(a, (unsigned)(t % 192)), r_pow2(a, (unsigned)(t % 192)));
   }
+  if (!bad) bad |= wide_sums(edge, ne, n_random / 400);
   if (!bad) printf("ok %ld\n", checked);
   return bad;
 }

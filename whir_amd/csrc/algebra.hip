@@ -694,7 +694,7 @@ __global__ __launch_bounds__(RT) void k_view_pass(const uint64_t* a_in, uint64_t
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int d = 0; d < D; d++) W[t][d] = gl::Wide{0, 0, 0};
-#pragma unroll 4
+#pragma unroll 1
     for (size_t l = threadIdx.x; l < B; l += blockDim.x) {
       const uint64_t x = a_in[base + l];
       Ext<D> lo[NT];

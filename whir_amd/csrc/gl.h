@@ -228,12 +228,19 @@ GL_FN Wide wide_prod(uint64_t a, uint64_t b) {
   w.top = 0;
   return w;
 }
+// The GPU forms below add and subtract 32 bits at a time through the carry flag
+// (v_add_co_u32 / v_addc_co_u32, full rate): a 64-bit __builtin_addcl compiles to a
+// half-rate 64-bit add and a half-rate 64-bit compare per word instead.
 GL_FN void wide_add(Wide& w, const Wide& x) {
 #if defined(GL_CARRY)
-  unsigned long c0, c1;
-  w.lo = __builtin_addcl(w.lo, x.lo, 0, &c0);
-  w.hi = __builtin_addcl(w.hi, x.hi, c0, &c1);
-  w.top += x.top + (uint32_t)c1;
+  uint32_t c;
+  const uint32_t w0 = __builtin_addc((uint32_t)w.lo, (uint32_t)x.lo, 0u, &c);
+  const uint32_t w1 = __builtin_addc((uint32_t)(w.lo >> 32), (uint32_t)(x.lo >> 32), c, &c);
+  const uint32_t w2 = __builtin_addc((uint32_t)w.hi, (uint32_t)x.hi, c, &c);
+  const uint32_t w3 = __builtin_addc((uint32_t)(w.hi >> 32), (uint32_t)(x.hi >> 32), c, &c);
+  w.top = __builtin_addc(w.top, x.top, c, &c);
+  w.lo = ((uint64_t)w1 << 32) | w0;
+  w.hi = ((uint64_t)w3 << 32) | w2;
 #else
   const unsigned __int128 s = (((unsigned __int128)w.hi << 64) | w.lo) + (((unsigned __int128)x.hi << 64) | x.lo);
   const bool c = (uint64_t)(s >> 64) < w.hi || ((uint64_t)(s >> 64) == w.hi && (uint64_t)s < w.lo);
@@ -245,10 +252,14 @@ GL_FN void wide_add(Wide& w, const Wide& x) {
 // w -= x, for w >= x as integers.
 GL_FN void wide_sub(Wide& w, const Wide& x) {
 #if defined(GL_CARRY)
-  unsigned long b0, b1;
-  w.lo = __builtin_subcl(w.lo, x.lo, 0, &b0);
-  w.hi = __builtin_subcl(w.hi, x.hi, b0, &b1);
-  w.top -= x.top + (uint32_t)b1;
+  uint32_t b;
+  const uint32_t w0 = __builtin_subc((uint32_t)w.lo, (uint32_t)x.lo, 0u, &b);
+  const uint32_t w1 = __builtin_subc((uint32_t)(w.lo >> 32), (uint32_t)(x.lo >> 32), b, &b);
+  const uint32_t w2 = __builtin_subc((uint32_t)w.hi, (uint32_t)x.hi, b, &b);
+  const uint32_t w3 = __builtin_subc((uint32_t)(w.hi >> 32), (uint32_t)(x.hi >> 32), b, &b);
+  w.top = __builtin_subc(w.top, x.top, b, &b);
+  w.lo = ((uint64_t)w1 << 32) | w0;
+  w.hi = ((uint64_t)w3 << 32) | w2;
 #else
   const unsigned __int128 a = ((unsigned __int128)w.hi << 64) | w.lo, b = ((unsigned __int128)x.hi << 64) | x.lo;
   w.top -= x.top + (a < b ? 1u : 0u);
@@ -257,6 +268,9 @@ GL_FN void wide_sub(Wide& w, const Wide& x) {
   w.hi = (uint64_t)(d >> 64);
 #endif
 }
+// w += a b.  (Forming the product with four independent v_mad_u64_u32 and adding its
+// halves in two carry chains, instead of the addend-chained product, measured slower
+// on MI355X: more registers and condition-register hazards.)
 GL_FN void wide_mac(Wide& w, uint64_t a, uint64_t b) { wide_add(w, wide_prod(a, b)); }
 // w <<= k for 1 <= k <= 3.
 GL_FN void wide_shl(Wide& w, unsigned k) {
