@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace whir {
@@ -29,6 +30,19 @@ __device__ __forceinline__ Ext<D> ld(const uint64_t* p, size_t i) {
   for (int d = 0; d < D; d++) r.c[d] = p[i * D + d];
   return r;
 }
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N - 1>{}):
+// a loop whose index is a compile-time constant in the body, so arrays indexed by it
+// stay in registers however large the body is (an unroll pragma gives up on large
+// bodies, and the arrays then go to scratch memory).
+template <class F, int... J>
+__device__ __forceinline__ void static_for_seq(F&& f, std::integer_sequence<int, J...>) {
+  (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
 template <int D>
 __device__ __forceinline__ void st(uint64_t* p, size_t i, const Ext<D>& v) {
 #pragma unroll
@@ -576,8 +590,8 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
   }
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < q; i += (size_t)gridDim.x * blockDim.x) {
     Ext<D> x[4], y[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
+    static_for<4>([&](auto kc) {  // a constant k keeps x and y in registers for Fp3 too
+      constexpr int k = decltype(kc)::value;
       const size_t t = k * q + i;
       if constexpr (FOLD) {
         x[k] = fold_twice<Ds, D>(a_in, t, L, r1, r2);
@@ -586,7 +600,7 @@ __global__ __launch_bounds__(RT) void k_sc_pair(const uint64_t* a_in, uint64_t* 
         x[k] = ldx<Ds, D>(a_in, t);
         y[k] = VIEW ? view_at<D>(v, t) : ld<D>(b, t);
       }
-    }
+    });
     if constexpr (FOLD) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -951,6 +965,8 @@ __global__ __launch_bounds__(RT) void k_cov(uint64_t* acc, size_t n, int accumul
   for (int k = 0; k < K; k++) sum[k] = gl::ext_zero<D>();
   if (L.np) {
     const size_t mask = L.nlo - 1;
+    // term and query loops unrolled so several iterations' loads share one wait
+#pragma unroll 2
     for (size_t p = 0; p < L.np; p++) {
       const uint64_t* hi = buf + L.hi + p * L.nhi * D;
       const uint64_t* lo = buf + L.lo + p * L.nlo * D;
@@ -961,6 +977,7 @@ __global__ __launch_bounds__(RT) void k_cov(uint64_t* acc, size_t n, int accumul
       }
     }
   }
+#pragma unroll 2
   for (size_t j = 0; j < L.m_e; j++) {
     Ext<D> p = gl::emul<D>(ld<D>(buf + L.T0, j * L.nb + b), ld<D>(buf + L.T1e, j * RT + threadIdx.x));
     const Ext<D> step = ld<D>(buf + L.Xe, j);
@@ -970,6 +987,7 @@ __global__ __launch_bounds__(RT) void k_cov(uint64_t* acc, size_t n, int accumul
       if (k + 1 < K) p = gl::emul<D>(p, step);
     }
   }
+#pragma unroll 4
   for (size_t j = 0; j < L.m_b; j++) {
     Ext<D> p = gl::escale<D>(ld<D>(buf + L.T0, (L.m_e + j) * L.nb + b), buf[L.T1b + j * RT + threadIdx.x]);
     const uint64_t step = buf[L.Xb + j];
@@ -1078,7 +1096,10 @@ __global__ __launch_bounds__(256) void k_geo(uint64_t* acc, size_t n, int accumu
 #pragma unroll
     for (int d = 0; d < D; d++) W[k][d] = gl::Wide{0, 0, 0};
   const uint64_t* hi = buf + L.hi;
-  // in-domain points: HI (extension) x LO (base), D products per row
+  // in-domain points: HI (extension) x LO (base), D products per row.  The query loops
+  // are unrolled so that the loads of several queries share one wait (short vectors run
+  // about one wave per SIMD, with nothing else to hide the latency).
+#pragma unroll 4
   for (size_t j = 0; j < L.m_b; j++) {
     const uint64_t lo = buf[L.lo_b + j * 256 + l];
     const uint64_t* hj = hi + ((L.m_e + j) * L.H + h0) * D;
@@ -1093,6 +1114,7 @@ __global__ __launch_bounds__(256) void k_geo(uint64_t* acc, size_t n, int accumu
 #pragma unroll
     for (int d = 0; d < D; d++) sum[k].c[d] = gl::wide_reduce(W[k][d]);
   // OOD points: full extension products
+#pragma unroll 2
   for (size_t j = 0; j < L.m_e; j++) {
     const Ext<D> lo = ld<D>(buf + L.lo_e, j * 256 + l);
 #pragma unroll
@@ -1490,16 +1512,23 @@ __global__ __launch_bounds__(RT) void k_uni(const uint64_t* c, size_t n, const u
 // with the launch, then steps by x_j^RT; one block stores its sums itself.
 static constexpr size_t UNI_SMALL_BLOCKS = 256;
 
-template <int Ds, int D>
-__global__ __launch_bounds__(RT) void k_uni_small(const uint64_t* c, size_t n, UniPow pw, int m, uint64_t* parts,
+template <int Ds, int D, int MP>
+__global__ __launch_bounds__(RT) void k_uni_small(const uint64_t* c, size_t n, UniPow pw, uint64_t* parts,
                                                   MailboxSig sig) {
   static_assert(RT == 256, "the step x^RT is the square x^(2^8)");
   const size_t i0 = blockIdx.x * (size_t)RT * UNI_K + threadIdx.x;
+  // Per point: x_j^i0 by square-and-multiply, then the lane's elements in order.  The
+  // elements are loaded a chunk at a time before the chunk's arithmetic (zero past the
+  // end), so a chunk's loads are in flight together rather than one per step of the
+  // power chain; Fp3 takes chunks of four (sixteen unrolled Fp3 steps do not fit the
+  // registers).
+  constexpr int UC = D == 3 ? 4 : UNI_K;
   Ext<D> acc[UNI_MP];
 #pragma unroll
-  for (int j = 0; j < UNI_MP; j++) {
-    acc[j] = gl::ext_zero<D>();
-    if (j < m && i0 < n) {
+  for (int j = 0; j < UNI_MP; j++) acc[j] = gl::ext_zero<D>();
+  if (i0 < n) {
+    static_for<MP>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
       Ext<D> p = gl::ext_base<D>(1);
       for (int bit = 0; (i0 >> bit) != 0; bit++)
         if ((i0 >> bit) & 1) {
@@ -1511,14 +1540,21 @@ __global__ __launch_bounds__(RT) void k_uni_small(const uint64_t* c, size_t n, U
       Ext<D> step;
 #pragma unroll
       for (int d = 0; d < D; d++) step.c[d] = pw.w[(j * UNI_PW + 8) * D + d];
-      for (int k = 0; k < UNI_K; k++) {
-        const size_t i = i0 + (size_t)k * RT;
-        if (i >= n) break;
-        if constexpr (Ds == 1) acc[j] = gl::eadd<D>(acc[j], gl::escale<D>(p, c[i]));
-        else acc[j] = gl::eadd<D>(acc[j], gl::emul<D>(p, ld<D>(c, i)));
-        p = gl::emul<D>(p, step);
+      for (int k0 = 0; k0 < UNI_K; k0 += UC) {
+        Ext<Ds> cv[UC];
+#pragma unroll
+        for (int k = 0; k < UC; k++) {
+          const size_t i = i0 + (size_t)(k0 + k) * RT;
+          cv[k] = i < n ? ldx<Ds, Ds>(c, i) : gl::ext_zero<Ds>();
+        }
+#pragma unroll
+        for (int k = 0; k < UC; k++) {
+          if constexpr (Ds == 1) acc[j] = gl::eadd<D>(acc[j], gl::escale<D>(p, cv[k].c[0]));
+          else acc[j] = gl::eadd<D>(acc[j], gl::emul<D>(p, cv[k]));
+          p = gl::emul<D>(p, step);
+        }
       }
-    }
+    });
   }
   block_reduce_store<D, UNI_MP>(acc, parts + (size_t)blockIdx.x * UNI_MP * D, sig);
 }
@@ -1562,8 +1598,17 @@ static void uni_t(const uint64_t* c, size_t n, const uint64_t* xs, size_t m, uin
     if (small) {
       {
         ProfScope prof(kernel_name("k_uni_small", Ds, D), s, 8.0 * Ds * (double)n);
-        k_uni_small<Ds, D><<<(unsigned)L.nb, RT, 0, s>>>(c, n, pw, (int)mj, L.nb == 1 ? dst : parts.as<uint64_t>(),
-                                                       L.nb == 1 && last ? mailbox_take(dst) : MailboxSig{});
+        uint64_t* o_small = L.nb == 1 ? dst : parts.as<uint64_t>();
+        const MailboxSig sig = L.nb == 1 && last ? mailbox_take(dst) : MailboxSig{};
+        auto go = [&](auto mp) {
+          k_uni_small<Ds, D, decltype(mp)::value><<<(unsigned)L.nb, RT, 0, s>>>(c, n, pw, o_small, sig);
+        };
+        switch (mj) {
+          case 1: go(std::integral_constant<int, 1>{}); break;
+          case 2: go(std::integral_constant<int, 2>{}); break;
+          case 3: go(std::integral_constant<int, 3>{}); break;
+          default: go(std::integral_constant<int, 4>{}); break;
+        }
         WHIR_HIP(hipGetLastError());
       }
       if (L.nb > 1) finish_reduce<D, UNI_MP>(parts.as<uint64_t>(), (unsigned)L.nb, dst, s, last);
@@ -1842,15 +1887,34 @@ __global__ void k_gather(const uint64_t* cw, size_t N, size_t M, const uint64_t*
 // each) and Merkle sibling hashes (out_hashes[i] = nodes[sib[i]]); idx_sib holds the
 // k row indices followed by the ns node indices.  With D > 0 it also sums each opened
 // row against per-column weights of degree D (wts[j], D words):
-// out_vals[q] = sum_j wts[j] cw[j*N + idx[q]].
+// out_vals[q] = sum_j wts[j] cw[j*N + idx[q]].  When M is a power of two up to 64 the
+// M lanes that gather a row each weight their element and sum across the row with
+// XOR shuffles (the row's lanes are aligned within one wave), so no lane waits on M
+// loads in turn; other widths sum each row on one lane.
 template <int Ds, int D>
 __global__ void k_gather_open(const uint64_t* __restrict__ cw, size_t N, size_t M, const uint64_t* __restrict__ idx_sib,
                               size_t k, const uint4* __restrict__ nodes, size_t ns, uint64_t* out_rows,
-                              uint4* out_hashes, const uint64_t* __restrict__ wts, uint64_t* out_vals) {
+                              uint4* out_hashes, const uint64_t* __restrict__ wts, uint64_t* out_vals,
+                              bool lane_sums) {
   size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e < k * M) {
     const size_t q = e / M, j = e % M, r = idx_sib[q];
-    if (r < N) st<Ds>(out_rows, e, ld<Ds>(cw, j * N + r));
+    const Ext<Ds> x = r < N ? ld<Ds>(cw, j * N + r) : gl::ext_zero<Ds>();
+    if (r < N) st<Ds>(out_rows, e, x);
+    if constexpr (D > 0) {
+      if (lane_sums) {
+        Ext<D> p;
+        if constexpr (Ds == 1) p = gl::escale<D>(ld<D>(wts, j), x.c[0]);
+        else p = gl::emul<D>(ld<D>(wts, j), x);
+        for (unsigned o = 1; o < (unsigned)M; o <<= 1) {
+          Ext<D> t;
+#pragma unroll
+          for (int d = 0; d < D; d++) t.c[d] = __shfl_xor(p.c[d], (int)o);
+          p = gl::eadd<D>(p, t);
+        }
+        if (j == 0) st<D>(out_vals, q, p);  // zero for an index past the codeword
+      }
+    }
     return;
   }
   e -= k * M;
@@ -1860,10 +1924,11 @@ __global__ void k_gather_open(const uint64_t* __restrict__ cw, size_t N, size_t 
   }
   if constexpr (D > 0) {
     e -= 2 * ns;
-    if (e < k) {
+    if (!lane_sums && e < k) {
       const size_t r = idx_sib[e];
       Ext<D> acc = gl::ext_zero<D>();
       if (r < N) {
+#pragma unroll 8
         for (size_t j = 0; j < M; j++) {
           if constexpr (Ds == 1) acc = gl::eadd<D>(acc, gl::escale<D>(ld<D>(wts, j), cw[j * N + r]));
           else acc = gl::eadd<D>(acc, gl::emul<D>(ld<D>(wts, j), ld<D>(cw, j * N + r)));
@@ -1878,10 +1943,14 @@ template <int Ds, int D>
 static void gather_open_t(const uint64_t* cw, size_t N, size_t M, const uint64_t* idx_sib, size_t k,
                           const uint8_t* nodes, size_t ns, uint64_t* out_rows, uint8_t* out_hashes,
                           const uint64_t* wts, uint64_t* out_vals, hipStream_t s) {
-  const size_t total = k * M + 2 * ns + (D > 0 ? k : 0);
+  const bool lane_sums = D > 0 && M >= 1 && M <= 64 && (M & (M - 1)) == 0;
+  const size_t total = k * M + 2 * ns + (D > 0 && !lane_sums ? k : 0);
   if (!total) return;
+  // Rows start at multiples of M within each block (RT % M == 0), so a row's lanes
+  // are aligned within one wave.
+  static_assert(RT % 64 == 0, "block size must be a multiple of the wave size");
   k_gather_open<Ds, D><<<blocks_for(total, RT), RT, 0, s>>>(cw, N, M, idx_sib, k, (const uint4*)nodes, ns, out_rows,
-                                                              (uint4*)out_hashes, wts, out_vals);
+                                                              (uint4*)out_hashes, wts, out_vals, lane_sums);
   WHIR_HIP(hipGetLastError());
 }
 
