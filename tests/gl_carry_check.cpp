@@ -1,6 +1,7 @@
 // CPU check of the GPU formulations of Goldilocks arithmetic (whir_amd/csrc/gl.h
 // with GL_HOST_CARRY_TEST: the carry-chain add/sub, the 4-product 128-bit multiply,
-// the carry-chain reduction, mul_pow2 and the 128-bit-plus-carries sums of products)
+// the carry-chain reduction, mul_pow2, the 128-bit-plus-carries sums of products and
+// the unreduced extension dot products)
 // against unsigned __int128 arithmetic,
 // over every pair of edge values and a stream of pseudo-random pairs.
 // Prints "ok <count>" or the first mismatch; exit status 0 iff all agree.
@@ -136,6 +137,51 @@ static int wide_sums(const uint64_t* vals, int nv, long rounds) {
   return bad | wide_check("wide_mac.ones", w, e);
 }
 
+// Dot products of extension vectors through emul_acc / ext_reduce against sums of the
+// per-product reductions, with terms of edge values (p - 1, ...) to push the carries.
+static int ext_sums(const uint64_t* vals, int nv, long rounds) {
+  int bad = 0;
+  for (long r = 0; r < rounds && !bad; r++) {
+    const int len = 1 + (int)(rnd() % 200);
+    gl::Wide w2[2], w3[3];
+    gl::wide_zero<2>(w2);
+    gl::wide_zero<3>(w3);
+    uint64_t e2[2] = {0, 0}, e3[3] = {0, 0, 0};
+    for (int i = 0; i < len; i++) {
+      uint64_t x[3], y[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = (rnd() & 1) ? canon(rnd()) : canon(vals[rnd() % nv]);
+        y[d] = (rnd() & 1) ? canon(rnd()) : canon(vals[rnd() % nv]);
+      }
+      gl::emul_acc<2>(w2, gl::Ext<2>{{x[0], x[1]}}, gl::Ext<2>{{y[0], y[1]}});
+      gl::emul_acc<3>(w3, gl::Ext<3>{{x[0], x[1], x[2]}}, gl::Ext<3>{{y[0], y[1], y[2]}});
+      uint64_t z2[2], z3[3];
+      r_emul2(x, y, z2);
+      r_emul3(x, y, z3);
+      for (int d = 0; d < 2; d++) e2[d] = r_add(e2[d], z2[d]);
+      for (int d = 0; d < 3; d++) e3[d] = r_add(e3[d], z3[d]);
+    }
+    const gl::Ext<2> g2 = gl::ext_reduce<2>(w2);
+    const gl::Ext<3> g3 = gl::ext_reduce<3>(w3);
+    for (int d = 0; d < 2; d++) bad |= check("emul_acc2", len, d, g2.c[d], e2[d]);
+    for (int d = 0; d < 3; d++) bad |= check("emul_acc3", len, d, g3.c[d], e3[d]);
+  }
+  // long runs of the largest coefficients: every term near 9 * 2^128 (Fp2, c0)
+  gl::Wide w2[2];
+  gl::wide_zero<2>(w2);
+  uint64_t e2[2] = {0, 0};
+  const uint64_t x[3] = {P - 1, P - 1, P - 1};
+  for (int i = 0; i < 5000 && !bad; i++) {
+    gl::emul_acc<2>(w2, gl::Ext<2>{{x[0], x[1]}}, gl::Ext<2>{{x[0], x[1]}});
+    uint64_t z2[2];
+    r_emul2(x, x, z2);
+    for (int d = 0; d < 2; d++) e2[d] = r_add(e2[d], z2[d]);
+  }
+  const gl::Ext<2> g2 = gl::ext_reduce<2>(w2);
+  for (int d = 0; d < 2; d++) bad |= check("emul_acc2.max", 5000, d, g2.c[d], e2[d]);
+  return bad;
+}
+
 int main(int argc, char** argv) {
   long n_random = argc > 1 ? atol(argv[1]) : 2000000;
   const uint64_t E = 0xFFFFFFFFull;
@@ -179,6 +225,7 @@ int main(int argc, char** argv) {
     if ((t & 63) == 0) bad |= check("mul_pow2", a, t % 192, gl::mul_pow2(a, (unsigned)(t % 192)), r_pow2(a, (unsigned)(t % 192)));
   }
   if (!bad) bad |= wide_sums(edge, ne, n_random / 400);
+  if (!bad) bad |= ext_sums(edge, ne, n_random / 2000);
   if (!bad) printf("ok %ld\n", checked);
   return bad;
 }

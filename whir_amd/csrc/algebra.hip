@@ -724,9 +724,12 @@ __global__ __launch_bounds__(RT) void k_view_pass(const uint64_t* a_in, uint64_t
 #pragma unroll
       for (int d = 0; d < D; d++) P[t].c[d] = gl::wide_reduce(W[t][d]);
   } else {
+    // extension x: the products' coefficients are summed unreduced, one reduction per
+    // coefficient per block
+    gl::Wide W[NT][D];
 #pragma unroll
-    for (int t = 0; t < NT; t++) P[t] = gl::ext_zero<D>();
-#pragma unroll 2
+    for (int t = 0; t < NT; t++) gl::wide_zero<D>(W[t]);
+#pragma unroll 1
     for (size_t l = threadIdx.x; l < B; l += blockDim.x) {
       const size_t p = base + l;
       Ext<D> lo[NT];
@@ -740,8 +743,10 @@ __global__ __launch_bounds__(RT) void k_view_pass(const uint64_t* a_in, uint64_t
         x = ldx<Ds, D>(a_in, p);
       }
 #pragma unroll
-      for (int t = 0; t < NT; t++) P[t] = gl::eadd<D>(P[t], gl::emul<D>(x, lo[t]));
+      for (int t = 0; t < NT; t++) gl::emul_acc<D>(W[t], x, lo[t]);
     }
+#pragma unroll
+    for (int t = 0; t < NT; t++) P[t] = gl::ext_reduce<D>(W[t]);
   }
   block_sum<D, NT>(P);
   if (threadIdx.x == 0) {

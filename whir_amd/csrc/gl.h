@@ -319,6 +319,56 @@ GL_FN Ext<3> emul<3>(const Ext<3>& a, const Ext<3>& b) {
   return r;
 }
 
+// Sums of extension products without a reduction per term: w[d] += (a b)_d as an
+// integer (the coefficient before reduction mod p), so a dot product reduces once per
+// coefficient at the end (ext_reduce).  Each term adds less than 9 * 2^128 per
+// coefficient, so the 32-bit carry word holds 2^28 terms.
+template <int D>
+GL_FN void emul_acc(Wide (&w)[D], const Ext<D>& a, const Ext<D>& b);
+template <>
+GL_FN void emul_acc<1>(Wide (&w)[1], const Ext<1>& a, const Ext<1>& b) {
+  wide_mac(w[0], a.c[0], b.c[0]);
+}
+// c0 += a0 b0 + 7 a1 b1 (as + 8 t - t, w >= 8t >= t), c1 += a0 b1 + a1 b0
+template <>
+GL_FN void emul_acc<2>(Wide (&w)[2], const Ext<2>& a, const Ext<2>& b) {
+  const Wide t = wide_prod(a.c[1], b.c[1]);
+  Wide t8 = t;
+  wide_shl(t8, 3);
+  wide_mac(w[0], a.c[0], b.c[0]);
+  wide_add(w[0], t8);
+  wide_sub(w[0], t);
+  wide_mac(w[1], a.c[0], b.c[1]);
+  wide_mac(w[1], a.c[1], b.c[0]);
+}
+// c0 += a0 b0 + 2 (a1 b2 + a2 b1), c1 += a0 b1 + a1 b0 + 2 a2 b2, c2 += a0 b2 + a1 b1 + a2 b0
+template <>
+GL_FN void emul_acc<3>(Wide (&w)[3], const Ext<3>& a, const Ext<3>& b) {
+  Wide t = wide_prod(a.c[1], b.c[2]);
+  wide_mac(t, a.c[2], b.c[1]);
+  wide_shl(t, 1);
+  wide_add(w[0], t);
+  wide_mac(w[0], a.c[0], b.c[0]);
+  Wide u = wide_prod(a.c[2], b.c[2]);
+  wide_shl(u, 1);
+  wide_add(w[1], u);
+  wide_mac(w[1], a.c[0], b.c[1]);
+  wide_mac(w[1], a.c[1], b.c[0]);
+  wide_mac(w[2], a.c[0], b.c[2]);
+  wide_mac(w[2], a.c[1], b.c[1]);
+  wide_mac(w[2], a.c[2], b.c[0]);
+}
+template <int D>
+GL_FN void wide_zero(Wide (&w)[D]) {
+  for (int d = 0; d < D; d++) w[d] = Wide{0, 0, 0};
+}
+template <int D>
+GL_FN Ext<D> ext_reduce(const Wide (&w)[D]) {
+  Ext<D> r;
+  for (int d = 0; d < D; d++) r.c[d] = wide_reduce(w[d]);
+  return r;
+}
+
 template <int D>
 GL_FN Ext<D> epow(Ext<D> a, uint64_t e) {
   Ext<D> r = ext_base<D>(1);
