@@ -289,7 +289,7 @@ def main():
         vj = json.load(open(VALU_FILE))
         for name, kern in POW_KERNELS.items():
             entry = vj["kernels"].get(kern)
-            if entry:
+            if entry and vj.get("pow_nonces"):
                 per_hash = entry["valu_per_launch"] * entry["launches"] / vj["pow_nonces"]
                 rate = per_hash * pow_rates[f"{name}_ghashes_per_s"] * 1e9 / 1e12
                 pow_rates[f"{name}_valu"] = {"wave_instructions_per_hash": round(per_hash, 3),
