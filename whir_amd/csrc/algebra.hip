@@ -885,6 +885,8 @@ struct CovPlan {
   size_t in_z, in_s, in_x, in_t;          // uploaded inputs (word offsets)
   size_t hi, lo, T0, T1e, Xe, T1b, Xb;    // tables (word offsets)
   size_t words, entries;
+  size_t in_sq;                           // per extension point: x^(S 2^k) for k < kb, then x^(2^k)
+  int kb;                                 // for k <= 8 (squares made on the host)
 };
 
 // Inputs of the covector tables (eq points and scalars, power points and scalars) up
@@ -933,22 +935,36 @@ __global__ void k_cov_tables(uint64_t* buf, CovPlan L, const uint64_t* in_dev, C
     return;
   }
   e -= sec[1];
+  // extension points take their powers from squares made on the host (a product per set
+  // bit of the exponent instead of a square-and-multiply chain per lane)
+  const size_t nsq = (size_t)L.kb + 9;
   if (e < sec[2]) {
     const size_t j = e / L.nb, b = e % L.nb;
-    const uint64_t ex = (uint64_t)b * L.S;
-    const Ext<D> t = in_ld<D>(in_dev, in_arg, L.in_t, j), x = in_ld<D>(in_dev, in_arg, L.in_x, j);
-    Ext<D> v = j < L.m_e ? gl::emul<D>(t, gl::epow<D>(x, ex)) : gl::escale<D>(t, gl::pow(x.c[0], ex));
+    const Ext<D> t = in_ld<D>(in_dev, in_arg, L.in_t, j);
+    Ext<D> v;
+    if (j < L.m_e) {
+      v = t;
+      for (int k = 0; k < L.kb; k++)
+        if ((b >> k) & 1) v = gl::emul<D>(v, in_ld<D>(in_dev, in_arg, L.in_sq, j * nsq + k));
+    } else {
+      v = gl::escale<D>(t, gl::pow(in_ld<D>(in_dev, in_arg, L.in_x, j).c[0], (uint64_t)b * L.S));
+    }
     st<D>(buf + L.T0, e, v);
     return;
   }
   e -= sec[2];
   if (e < sec[3]) {
-    st<D>(buf + L.T1e, e, gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, e / RT), e % RT));
+    const size_t j = e / RT, r = e % RT;
+    Ext<D> v = gl::ext_base<D>(1);
+    for (int k = 0; k < 8; k++)
+      if ((r >> k) & 1) v = gl::emul<D>(v, in_ld<D>(in_dev, in_arg, L.in_sq, j * nsq + L.kb + k));
+    st<D>(buf + L.T1e, e, v);
     return;
   }
   e -= sec[3];
   if (e < sec[4]) {
-    st<D>(buf + L.Xe, e, gl::epow<D>(in_ld<D>(in_dev, in_arg, L.in_x, e), RT));
+    static_assert(RT == 256, "x^RT is the square x^(2^8)");
+    st<D>(buf + L.Xe, e, in_ld<D>(in_dev, in_arg, L.in_sq, e * nsq + L.kb + 8));
     return;
   }
   e -= sec[4];
@@ -1048,6 +1064,8 @@ struct GeoPlan {
   size_t hi, lo_e, lo_b;        // tables: HI (m H elements), LO of OOD points (m_e 256
                                 // elements), LO of in-domain points (m_b 256 words)
   uint64_t N;                   // domain size of the in-domain points (a power of two)
+  size_t in_sq;                 // inputs: per OOD point, x^(256 2^k) for k < kh, then x^(2^k)
+  int kh;                       // for k < 8 (squares made on the host)
 };
 
 template <int D>
@@ -1070,11 +1088,13 @@ __global__ void k_geo_tables(uint64_t* buf, GeoPlan L, const uint64_t* in_dev, C
     return;
   }
   e -= sec[1];
-  if (e < sec[2]) {  // OOD HI in runs: t_j x_j^(256 h)
+  const size_t nsq = (size_t)L.kh + 8;  // squares per OOD point
+  if (e < sec[2]) {  // OOD HI in runs: t_j x_j^(256 h), x^(256 h0) from the squares of x^256
     const size_t j = e / runs_hi, h0 = (e % runs_hi) * GEO_RUN;
-    const Ext<D> x = in_ld<D>(in_dev, in_arg, L.in_x, j);
-    const Ext<D> step = gl::epow<D>(x, 256);
-    Ext<D> v = gl::emul<D>(in_ld<D>(in_dev, in_arg, L.in_t, j), gl::epow<D>(x, (uint64_t)h0 * 256));
+    const Ext<D> step = in_ld<D>(in_dev, in_arg, L.in_sq, j * nsq);
+    Ext<D> v = in_ld<D>(in_dev, in_arg, L.in_t, j);
+    for (int k = 0; k < L.kh; k++)
+      if ((h0 >> k) & 1) v = gl::emul<D>(v, in_ld<D>(in_dev, in_arg, L.in_sq, j * nsq + k));
     for (size_t h = h0; h < h0 + GEO_RUN && h < L.H; h++) {
       st<D>(buf + L.hi, j * L.H + h, v);
       v = gl::emul<D>(v, step);
@@ -1085,7 +1105,9 @@ __global__ void k_geo_tables(uint64_t* buf, GeoPlan L, const uint64_t* in_dev, C
   if (e >= sec[3]) return;
   const size_t j = e / (256 / GEO_RUN), l0 = (e % (256 / GEO_RUN)) * GEO_RUN;  // OOD LO in runs: x_j^l
   const Ext<D> x = in_ld<D>(in_dev, in_arg, L.in_x, j);
-  Ext<D> v = gl::epow<D>(x, l0);
+  Ext<D> v = gl::ext_base<D>(1);
+  for (int k = 0; k < 8; k++)
+    if ((l0 >> k) & 1) v = gl::emul<D>(v, in_ld<D>(in_dev, in_arg, L.in_sq, j * nsq + L.kh + k));
   for (size_t l = l0; l < l0 + GEO_RUN; l++) {
     st<D>(buf + L.lo_e, j * 256 + l, v);
     v = gl::emul<D>(v, x);
@@ -1148,6 +1170,25 @@ static void geo_terms_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t
   in.insert(in.end(), scalars, scalars + m * D);
   L.in_e = in.size();
   in.insert(in.end(), exps, exps + m_b);
+  // Squares for the OOD tables: x^(256 2^k) for k < kh (2^kh >= H) and x^(2^k) for k < 8,
+  // so a lane forms its run's first power from the bits of its index.
+  L.kh = 0;
+  while (((size_t)1 << L.kh) < L.H) L.kh++;
+  L.in_sq = in.size();
+  for (size_t j = 0; j < m_e; j++) {
+    Ext<D> x;
+    for (int d = 0; d < D; d++) x.c[d] = ext_points[j * D + d];
+    Ext<D> lo[8];
+    for (int k = 0; k < 8; k++) {
+      lo[k] = x;
+      x = gl::emul<D>(x, x);
+    }
+    for (int k = 0; k < L.kh; k++) {  // x is x_j^256 here
+      in.insert(in.end(), x.c, x.c + D);
+      x = gl::emul<D>(x, x);
+    }
+    for (int k = 0; k < 8; k++) in.insert(in.end(), lo[k].c, lo[k].c + D);
+  }
   const bool by_arg = in.size() <= COV_ARG_WORDS;
   size_t w = by_arg ? 0 : in.size();
   L.hi = w;   w += m * L.H * D;
@@ -1216,6 +1257,24 @@ static void covector_t(uint64_t* acc, size_t n, bool accumulate, const uint64_t*
   L.in_t = in.size();
   for (size_t j : order_e) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
   for (size_t j : order_b) in.insert(in.end(), scalars + j * D, scalars + (j + 1) * D);
+  // Squares of the extension points: x^(S 2^k) for k < kb (2^kb >= nb), then x^(2^k)
+  // for k <= 8.
+  L.kb = 0;
+  while (((size_t)1 << L.kb) < L.nb) L.kb++;
+  L.in_sq = in.size();
+  for (size_t j : order_e) {
+    Ext<D> x;
+    for (int d = 0; d < D; d++) x.c[d] = points[j * D + d];
+    Ext<D> xs = gl::epow<D>(x, L.S);
+    for (int k = 0; k < L.kb; k++) {
+      in.insert(in.end(), xs.c, xs.c + D);
+      xs = gl::emul<D>(xs, xs);
+    }
+    for (int k = 0; k <= 8; k++) {
+      in.insert(in.end(), x.c, x.c + D);
+      x = gl::emul<D>(x, x);
+    }
+  }
   // Small inputs go with the launch; larger ones are copied to the front of the buffer.
   const bool by_arg = in.size() <= COV_ARG_WORDS;
   size_t w = by_arg ? 0 : in.size();
