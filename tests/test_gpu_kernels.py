@@ -39,6 +39,8 @@ RS_CASES = [
     (1, 3, 1 << 12, 1 << 13, 0), (1, 2, 3000, 1 << 13, 0), (1, 16, 1 << 13, 1 << 15, 0),
     (2, 3, 8, 32, 0), (2, 16, 1 << 9, 1 << 14, 0), (2, 2, 1 << 12, 1 << 14, 0), (2, 4, 100, 256, 3),
     (3, 3, 8, 32, 0), (3, 16, 1 << 8, 1 << 12, 0), (3, 2, 1 << 12, 1 << 13, 0),
+    # one-coefficient messages (a broadcast, no transform), and one with a mask (transform)
+    (1, 3, 1, 64, 0), (2, 16, 1, 1 << 14, 0), (3, 4, 1, 256, 0), (2, 4, 1, 64, 2),
 ]
 
 

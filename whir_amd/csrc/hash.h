@@ -30,6 +30,12 @@ void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, 
 // level's node is the hash of the level below paired with itself).
 void commit_const_rows_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes,
                            hipStream_t st, uint8_t* root_out = nullptr);
+// The same commit with the root computed on the host (BLAKE3 rows of at most one chunk,
+// see const_rows_on_host): root receives it when the call returns; the node array is
+// filled by a launch on st.
+bool const_rows_on_host(int engine, size_t row_bytes, size_t N);
+void commit_const_rows_host(int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                            uint8_t root[32]);
 
 // commit_cols_dev of `count` codewords of the same shape in shared launches: codeword
 // i at cw_tab[i], its nodes at nodes_tab[i] (device arrays of pointers), its root to

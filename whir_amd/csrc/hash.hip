@@ -22,6 +22,7 @@
 #include "hash_dev.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace whir {
 
@@ -759,6 +760,127 @@ void commit_const_rows_dev(int engine, int D, const uint64_t* cw, size_t N, size
   ProfScope prof("k_const_fill", st, 32.0 * (2.0 * N - 1));
   k_const_fill<<<(unsigned)std::min<size_t>(blocks_for(2 * N - 1, 256), 4096), 256, 0, st>>>(nodes, N,
                                                                                              dig.as<uint32_t>());
+  WHIR_HIP(hipGetLastError());
+}
+
+// Constant rows with the root made on the host (BLAKE3, rows of one chunk): the tree's
+// digests are a chain of ~20 dependent compressions, which one lane takes ~1.7 us each to
+// issue but the host runs in well under a microsecond each.  Row 0 comes to the host
+// through the stream's mailbox (one wait, which the caller needs for the root anyway);
+// the node array is then filled from the digests passed with the launch.
+static constexpr int CONST_MAX_LEVELS = 40;
+struct ConstDigests {
+  uint32_t w[CONST_MAX_LEVELS + 1][8];
+};
+
+__global__ void k_const_fill_arg(uint8_t* __restrict__ nodes, size_t W, ConstDigests dig) {
+  const size_t total = 2 * W - 1;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    size_t base = 0, width = W;  // level l covers [base, base + width)
+    int l = 0;
+    while (e >= base + width) {
+      base += width;
+      width >>= 1;
+      l++;
+    }
+    store_digest(nodes + 32 * e, dig.w[l]);
+  }
+}
+
+// Row 0 of the column-major codeword (word t = cw[(t / D) N D + t % D]) into the mailbox.
+__global__ void k_row0_mailbox(const uint64_t* __restrict__ cw, size_t N, int D, size_t words, uint64_t* out,
+                               MailboxSig sig) {
+  for (size_t t = threadIdx.x; t < words; t += blockDim.x) out[t] = cw[(t / D) * N * D + t % D];
+  __threadfence_system();  // every writer's stores reach host memory before the flag
+  __syncthreads();
+  if (threadIdx.x == 0) mailbox_raise(sig);
+}
+
+// BLAKE3 compression on the host (the public specification: 7 rounds of the G mixing
+// function over columns then diagonals, message words permuted between rounds).
+static void b3_compress_host(uint32_t cv[8], const uint32_t msg[16], uint32_t block_len, uint32_t flags) {
+  static const uint32_t IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+  static const int PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+  auto rotr = [](uint32_t x, int n) { return (x >> n) | (x << (32 - n)); };
+  uint32_t v[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
+                    IV[0], IV[1], IV[2], IV[3], 0u, 0u, block_len, flags};
+  uint32_t m[16];
+  std::copy(msg, msg + 16, m);
+  auto g = [&](int a, int b, int c, int d, uint32_t x, uint32_t y) {
+    v[a] = v[a] + v[b] + x;
+    v[d] = rotr(v[d] ^ v[a], 16);
+    v[c] = v[c] + v[d];
+    v[b] = rotr(v[b] ^ v[c], 12);
+    v[a] = v[a] + v[b] + y;
+    v[d] = rotr(v[d] ^ v[a], 8);
+    v[c] = v[c] + v[d];
+    v[b] = rotr(v[b] ^ v[c], 7);
+  };
+  for (int r = 0; r < 7; r++) {
+    g(0, 4, 8, 12, m[0], m[1]);
+    g(1, 5, 9, 13, m[2], m[3]);
+    g(2, 6, 10, 14, m[4], m[5]);
+    g(3, 7, 11, 15, m[6], m[7]);
+    g(0, 5, 10, 15, m[8], m[9]);
+    g(1, 6, 11, 12, m[10], m[11]);
+    g(2, 7, 8, 13, m[12], m[13]);
+    g(3, 4, 9, 14, m[14], m[15]);
+    if (r < 6) {
+      uint32_t t[16];
+      for (int i = 0; i < 16; i++) t[i] = m[PERM[i]];
+      std::copy(t, t + 16, m);
+    }
+  }
+  for (int i = 0; i < 8; i++) cv[i] = v[i] ^ v[i + 8];
+}
+
+bool const_rows_on_host(int engine, size_t row_bytes, size_t N) {
+  return engine == ENG_BLAKE3 && row_bytes > 0 && row_bytes <= 1024 && N > 0 && (N & (N - 1)) == 0 &&
+         (int)merkle_layers(N) <= CONST_MAX_LEVELS;
+}
+
+void commit_const_rows_host(int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                            uint8_t root[32]) {
+  const size_t words = M * D, bytes = 8 * words;
+  WHIR_REQUIRE(const_rows_on_host(ENG_BLAKE3, bytes, N), "commit: constant rows on the host need BLAKE3 rows of one chunk");
+  uint64_t* mbox = result_mailbox(st);
+  mailbox_arm(st);
+  k_row0_mailbox<<<1, 128, 0, st>>>(cw, N, D, words, mbox, mailbox_take(mbox));
+  WHIR_HIP(hipGetLastError());
+  mailbox_wait(st);
+  // Leaf: BLAKE3 of the row's bytes (one chunk: CHUNK_START on its first block,
+  // CHUNK_END | ROOT on its last), words little-endian.
+  static const uint32_t IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+  ConstDigests dig{};
+  uint32_t h[8];
+  std::copy(IV, IV + 8, h);
+  const size_t nblocks = (bytes + 63) / 64;
+  for (size_t b = 0; b < nblocks; b++) {
+    uint32_t m[16] = {};
+    const size_t len = std::min<size_t>(64, bytes - 64 * b);
+    for (size_t i = 0; i < len / 8; i++) {
+      const uint64_t w = mbox[8 * b + i];
+      m[2 * i] = (uint32_t)w;
+      m[2 * i + 1] = (uint32_t)(w >> 32);
+    }
+    const uint32_t flags = (b == 0 ? 1u : 0u) | (b + 1 == nblocks ? 2u | 8u : 0u);
+    b3_compress_host(h, m, (uint32_t)len, flags);
+  }
+  std::copy(h, h + 8, dig.w[0]);
+  // Level l's nodes all hash the 64-byte pair (d, d) of level l - 1's digest d.
+  const int levels = (int)merkle_layers(N);
+  for (int l = 1; l <= levels; l++) {
+    uint32_t m[16];
+    for (int i = 0; i < 8; i++) m[i] = m[8 + i] = h[i];
+    std::copy(IV, IV + 8, h);
+    b3_compress_host(h, m, 64, 1u | 2u | 8u);
+    std::copy(h, h + 8, dig.w[l]);
+  }
+  memcpy(root, h, 32);
+  ProfScope prof("k_const_fill", st, 32.0 * (2.0 * N - 1));
+  k_const_fill_arg<<<(unsigned)std::min<size_t>(blocks_for(2 * N - 1, 256), 4096), 256, 0, st>>>(nodes, N, dig);
   WHIR_HIP(hipGetLastError());
 }
 

@@ -950,8 +950,32 @@ static NttSched sched_for(int D, uint64_t N, int logC, uint64_t M, bool* found =
   return it != g_sched.end() ? it->second.s : default_sched(D, logC);
 }
 
+// Messages of one coefficient and no mask: the polynomial is a constant, so every
+// evaluation is that coefficient (WHIR's last commitment); a broadcast replaces the
+// transform.  Thread e writes element e % N of codeword e / N.
+template <int D>
+__global__ void k_rs_const(RsArgs a, int logN) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= (a.M << logN)) return;
+  const size_t j = e >> logN, r = e & (a.N - 1);  // N is a power of two
+  const uint64_t* m = msg_ptr<D>(a, j);
+  uint64_t* o = out_ptr<D>(a, j) + r * D;
+  if constexpr (D == 2) {
+    *(uint4*)o = make_uint4((uint32_t)m[0], (uint32_t)(m[0] >> 32), (uint32_t)m[1], (uint32_t)(m[1] >> 32));
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; d++) o[d] = m[d];
+  }
+}
+
 template <int D>
 static void launch(RsArgs a, uint64_t* scratch, hipStream_t st) {
+  if (a.L == 1 && a.mask_len == 0) {
+    ProfScope prof(kernel_name("k_rs_const", D), st, 8.0 * D * (double)a.M * (double)(a.N + 1));
+    k_rs_const<D><<<blocks_for(a.M * a.N, 256), 256, 0, st>>>(a, log2_exact(a.N));
+    WHIR_HIP(hipGetLastError());
+    return;
+  }
   WHIR_REQUIRE(a.logC <= 2 * MAX_PASS_LOG, "interleaved_encode: coset size above 2^22 is not supported");
   bool found = false;
   const NttSched s = sched_for(D, a.N, a.logC, a.M, &found);

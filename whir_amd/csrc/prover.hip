@@ -236,16 +236,21 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   }
   size_t nn = merkle_num_nodes(N);
   w.nodes = DevBuf(32 * nn, s);
-  // The root lands in the stream's pinned mailbox from the last tree launch.
   uint64_t* mbox = result_mailbox(s);
-  mailbox_arm(s);
-  if (L == 1 && (N & (N - 1)) == 0)  // every column is a constant: all rows are equal
-    commit_const_rows_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
-  else
-    commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
-  mailbox_wait(s);
   uint8_t root[32];
-  memcpy(root, mbox, 32);
+  const bool const_rows = L == 1 && (N & (N - 1)) == 0;  // every column is a constant: all rows are equal
+  if (const_rows && const_rows_on_host(engine, 8 * M * Ds, N)) {
+    commit_const_rows_host(Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, root);
+  } else {
+    // The root lands in the stream's pinned mailbox from the last tree launch.
+    mailbox_arm(s);
+    if (const_rows)
+      commit_const_rows_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
+    else
+      commit_cols_dev(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, (uint8_t*)mbox);
+    mailbox_wait(s);
+    memcpy(root, mbox, 32);
+  }
   t.prover_hash(root);
   w.ood_points.clear();
   w.ood_values.clear();
