@@ -200,6 +200,10 @@ thread_local struct CopyStream {
 // usual interleaving depth 16).
 constexpr size_t UPLOAD_CHUNK_MESSAGES = 4;
 
+// Out-of-domain evaluations of at most this many coefficient-point products run on the
+// host (Horner's rule over coefficients read through the mailbox).
+constexpr size_t HOST_OOD_TERMS = 512;
+
 template <int D>
 void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* const* vecs, size_t nvec, int Ds,
                 DevWitness& w, hipStream_t s, const HostUpload* up = nullptr) {
@@ -256,7 +260,33 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   w.ood_values.clear();
   for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
   const size_t nood = c.out_domain_samples;
-  if (nood) {
+  if (nood && nvec * nood * c.vector_size <= HOST_OOD_TERMS && nvec * c.vector_size * Ds <= RESULT_MAILBOX_WORDS) {
+    // Short vectors: their coefficients come to the host through the mailbox (the one wait
+    // the values needed anyway) and Horner's rule runs there, instead of an evaluation
+    // launch that one block would spend ~15 us on.
+    const size_t n = c.vector_size;
+    mailbox_arm(s);
+    for (size_t b = 0; b < nvec; b++)
+      WHIR_HIP(hipMemcpyAsync(mbox + b * n * Ds, vecs[b], n * Ds * 8, hipMemcpyDefault, s));
+    mailbox_wait(s);
+    w.ood_values.resize(nood * nvec * D);
+    for (size_t i = 0; i < nood; i++) {
+      gl::Ext<D> x;
+      for (int d = 0; d < D; d++) x.c[d] = w.ood_points[i * D + d];
+      for (size_t b = 0; b < nvec; b++) {
+        const uint64_t* cf = mbox + b * n * Ds;
+        gl::Ext<D> v = gl::ext_zero<D>();
+        for (size_t k = n; k-- > 0;) {
+          gl::Ext<D> ck;
+          if (Ds == 1) ck = gl::ext_base<D>(cf[k]);
+          else for (int d = 0; d < D; d++) ck.c[d] = cf[k * Ds + d];
+          v = gl::eadd<D>(gl::emul<D>(v, x), ck);
+        }
+        std::copy(v.c, v.c + D, &w.ood_values[(i * nvec + b) * D]);
+        t.prover_field(D, &w.ood_values[(i * nvec + b) * D], 1);
+      }
+    }
+  } else if (nood) {
     // All samples of a vector in shared passes over it, written to the mailbox
     // (vector b at words [b*pad*D, (b+1)*pad*D)); sent point-major.
     const size_t pad = (nood + 3) / 4 * 4;
