@@ -201,8 +201,27 @@ thread_local struct CopyStream {
 constexpr size_t UPLOAD_CHUNK_MESSAGES = 4;
 
 // Out-of-domain evaluations of at most this many coefficient-point products run on the
-// host (Horner's rule over coefficients read through the mailbox).
+// host (Horner's rule over coefficients copied to pinned host memory).
 constexpr size_t HOST_OOD_TERMS = 512;
+
+// Pinned host memory for short vectors that the host works on (out-of-domain samples and
+// short sumchecks read them here, and folded vectors go back from here), one buffer per
+// host thread (each worker thread drives its own stream).  It is not the result mailbox:
+// results queued ahead (an opening's rows) may still be there.  Copies into it are
+// followed by a mailbox wait; a later use on the thread rewrites it only after such a
+// wait, by which time the stream has run the copies that read it.
+constexpr size_t HOST_STAGE_WORDS = 2048;
+struct HostStage {
+  uint64_t* p = nullptr;
+  ~HostStage() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+static uint64_t* host_stage() {
+  thread_local HostStage st;
+  if (!st.p) WHIR_HIP(hipHostMalloc((void**)&st.p, HOST_STAGE_WORDS * 8, hipHostMallocDefault));
+  return st.p;
+}
 
 template <int D>
 void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* const* vecs, size_t nvec, int Ds,
@@ -260,21 +279,22 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   w.ood_values.clear();
   for (size_t i = 0; i < c.out_domain_samples; i++) push<D>(w.ood_points, verifier_field<D>(t));
   const size_t nood = c.out_domain_samples;
-  if (nood && nvec * nood * c.vector_size <= HOST_OOD_TERMS && nvec * c.vector_size * Ds <= RESULT_MAILBOX_WORDS) {
-    // Short vectors: their coefficients come to the host through the mailbox (the one wait
-    // the values needed anyway) and Horner's rule runs there, instead of an evaluation
-    // launch that one block would spend ~15 us on.
+  if (nood && nvec * nood * c.vector_size <= HOST_OOD_TERMS && nvec * c.vector_size * Ds <= HOST_STAGE_WORDS) {
+    // Short vectors: their coefficients come to the host (one wait, which the values
+    // needed anyway) and Horner's rule runs there, instead of an evaluation launch that
+    // one block would spend ~15 us on.
     const size_t n = c.vector_size;
+    uint64_t* stage = host_stage();
     mailbox_arm(s);
     for (size_t b = 0; b < nvec; b++)
-      WHIR_HIP(hipMemcpyAsync(mbox + b * n * Ds, vecs[b], n * Ds * 8, hipMemcpyDefault, s));
+      WHIR_HIP(hipMemcpyAsync(stage + b * n * Ds, vecs[b], n * Ds * 8, hipMemcpyDefault, s));
     mailbox_wait(s);
     w.ood_values.resize(nood * nvec * D);
     for (size_t i = 0; i < nood; i++) {
       gl::Ext<D> x;
       for (int d = 0; d < D; d++) x.c[d] = w.ood_points[i * D + d];
       for (size_t b = 0; b < nvec; b++) {
-        const uint64_t* cf = mbox + b * n * Ds;
+        const uint64_t* cf = stage + b * n * Ds;
         gl::Ext<D> v = gl::ext_zero<D>();
         for (size_t k = n; k-- > 0;) {
           gl::Ext<D> ck;
@@ -622,11 +642,74 @@ std::vector<E<D>> sumcheck_prove_pairs(Transcript& t, const SumcheckCfg& sc, int
 // work ahead of the sumcheck (an opening's gather) reads its results there.
 //
 // view: b given as a covector view of length n (see sumcheck_prove_pairs).
+// Sumchecks over at most this many elements run on the host: both vectors come over in
+// one wait, every round's sums, challenge and fold are host arithmetic (a few hundred
+// extension products at most), and the folded vectors go back with one copy.  On the
+// GPU such a sumcheck is four or five one-block launches of 4-16 us each with a host
+// round trip per one or two rounds.
+constexpr size_t HOST_SUMCHECK_MAX = 64;
+
+// sumcheck::Config::prove on the host for short vectors (a and b of length n in device
+// memory, extension elements): the same messages, challenges and folds as the device
+// rounds (x'_i = x_i + r (x_(h+i) - x_i), c0 = sum x_i y_i, c2 = sum dx_i dy_i over
+// i < h = n/2).
+template <int D>
+std::vector<E<D>> sumcheck_prove_host(Transcript& t, const SumcheckCfg& sc, int engine, uint64_t* a, uint64_t* b,
+                                      size_t& n, E<D>& sum, hipStream_t s, const std::function<void()>& after_first) {
+  const size_t words = n * D;
+  WHIR_REQUIRE(n <= HOST_SUMCHECK_MAX && 2 * words <= HOST_STAGE_WORDS, "sumcheck: vectors too long for the host path");
+  uint64_t* stage = host_stage();
+  mailbox_arm(s);
+  WHIR_HIP(hipMemcpyAsync(stage, a, words * 8, hipMemcpyDefault, s));
+  WHIR_HIP(hipMemcpyAsync(stage + words, b, words * 8, hipMemcpyDefault, s));
+  mailbox_wait(s);  // (results queued ahead, e.g. an opening's rows, are in the mailbox)
+  if (after_first) after_first();
+  std::vector<E<D>> x(n), y(n);
+  for (size_t i = 0; i < n; i++) {
+    x[i] = ld<D>(stage + i * D);
+    y[i] = ld<D>(stage + words + i * D);
+  }
+  std::vector<E<D>> chals;
+  size_t len = n;
+  for (size_t round = 0; round < sc.num_rounds; round++) {
+    const size_t h = len / 2;
+    E<D> c0 = gl::ext_zero<D>(), c2 = gl::ext_zero<D>();
+    for (size_t i = 0; i < h; i++) {
+      c0 = gl::eadd<D>(c0, gl::emul<D>(x[i], y[i]));
+      c2 = gl::eadd<D>(c2, gl::emul<D>(gl::esub<D>(x[h + i], x[i]), gl::esub<D>(y[h + i], y[i])));
+    }
+    const E<D> c1 = gl::esub<D>(gl::esub<D>(gl::esub<D>(sum, c0), c0), c2);
+    t.prover_field(D, c0.c, 1);
+    t.prover_field(D, c2.c, 1);
+    pow_prove(t, engine, sc.round_pow, s);
+    const E<D> r = verifier_field<D>(t);
+    chals.push_back(r);
+    sum = gl::eadd<D>(gl::emul<D>(gl::eadd<D>(gl::emul<D>(c2, r), c1), r), c0);
+    for (size_t i = 0; i < h; i++) {
+      x[i] = gl::eadd<D>(x[i], gl::emul<D>(r, gl::esub<D>(x[h + i], x[i])));
+      y[i] = gl::eadd<D>(y[i], gl::emul<D>(r, gl::esub<D>(y[h + i], y[i])));
+    }
+    len = h;
+  }
+  // The folded vectors back to the device.
+  for (size_t i = 0; i < len; i++) {
+    std::copy(x[i].c, x[i].c + D, stage + i * D);
+    std::copy(y[i].c, y[i].c + D, stage + (len + i) * D);
+  }
+  WHIR_HIP(hipMemcpyAsync(a, stage, len * D * 8, hipMemcpyDefault, s));
+  WHIR_HIP(hipMemcpyAsync(b, stage + len * D, len * D * 8, hipMemcpyDefault, s));
+  n = len;
+  return chals;
+}
+
 template <int D>
 std::vector<E<D>> sumcheck_prove(Transcript& t, const SumcheckCfg& sc, int engine, const uint64_t* a_src, int a_Ds,
                                  uint64_t* a, uint64_t* b, size_t& n, E<D>& sum, hipStream_t s,
                                  const std::function<void()>& after_first = {}, const CovView* view = nullptr) {
   WHIR_REQUIRE(n == sc.initial_size, "sumcheck: vector length does not match the configuration");
+  if (n <= HOST_SUMCHECK_MAX && n >= 2 && (n & (n - 1)) == 0 && sc.num_rounds >= 1 && !view && a_src == a &&
+      a_Ds == D && ((size_t)1 << sc.num_rounds) <= n)
+    return sumcheck_prove_host<D>(t, sc, engine, a, b, n, sum, s, after_first);
   if (n >= 16 && (n & (n - 1)) == 0 && sc.num_rounds >= 2)
     return sumcheck_prove_pairs<D>(t, sc, engine, a_src, a_Ds, a, b, n, sum, s, after_first, view);
   if (view) covview_write_dev(D, *view, b, n, s);
