@@ -34,6 +34,11 @@ void commit_const_rows_dev(int engine, int D, const uint64_t* cw, size_t N, size
 // see const_rows_on_host): root receives it when the call returns; the node array is
 // filled by a launch on st.
 bool const_rows_on_host(int engine, size_t row_bytes, size_t N);
+// commit_cols_dev with the top of the tree (at most 64 nodes wide) hashed on the host,
+// for the Keccak engines (host_top_ok): root receives the root when the call returns.
+bool host_top_ok(int engine);
+void commit_cols_host_top(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                          uint8_t root[32]);
 void commit_const_rows_host(int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
                             uint8_t root[32]);
 

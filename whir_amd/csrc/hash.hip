@@ -20,6 +20,7 @@
 
 #include <type_traits>
 #include "hash_dev.h"
+#include "keccak_host.h"
 
 #include <algorithm>
 #include <cstring>
@@ -399,13 +400,16 @@ static constexpr size_t MERKLE_WIDE_MIN = (size_t)1 << 19;
 
 // Merkle levels above a level of width w (a power of two) already in place.  The
 // root also goes to root_out (32 bytes) when given.  With a batch table, the same
-// levels of `count` trees (root i to tab.roots + 32 i).
-static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st, uint8_t* root_out,
-                        BatchTab tab = {}, size_t count = 1) {
-  if (w <= 1) {
+// levels of `count` trees (root i to tab.roots + 32 i).  With stop > 1 the launches end
+// at a level of width <= stop (never below it unless w already is), whose width is
+// returned; the caller makes the rest.
+static size_t merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStream_t st, uint8_t* root_out,
+                          BatchTab tab = {}, size_t count = 1, size_t stop = 1) {
+  if (w <= stop) {
+    if (stop > 1) return w;
     WHIR_REQUIRE(!tab.nodes, "merkle: a batch of one-leaf trees is not supported");
     if (root_out) WHIR_HIP(hipMemcpyAsync(root_out, nodes + 32 * (2 * W - 2), 32, hipMemcpyDefault, st));
-    return;
+    return w;
   }
   while (w >= MERKLE_WIDE_MIN) {
     const size_t blocks = w / ((size_t)MERKLE_THREADS << 3);
@@ -417,13 +421,13 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     WHIR_HIP(hipGetLastError());
     w >>= 3;
   }
-  while (w > 1) {
+  while (w > stop) {
     // Wide levels: eight nodes per lane first (2^11 per block); otherwise pairs
     // (2^9 per block), which has the shortest chain of dependent hashes.
     const int G1 = w >= ((size_t)MERKLE_THREADS << MERKLE_SPAN_LOG) ? merkle_gmax_rt(engine) : 1;
     const int maxL = MERKLE_SPAN_LOG - 3 + G1;
     int L = 0;
-    while (L < maxL && ((size_t)2 << L) <= w) L++;
+    while (L < maxL && ((size_t)2 << L) <= w && (w >> (L + 1)) >= stop) L++;
     const size_t blocks = w >> L;
     const unsigned threads =
         (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
@@ -439,6 +443,7 @@ static void merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStrea
     WHIR_HIP(hipGetLastError());
     w >>= L;
   }
+  return w;
 }
 
 void merkle_build_dev(int engine, uint8_t* nodes, size_t num_leaves, hipStream_t st, uint8_t* root_out) {
@@ -652,6 +657,55 @@ void commit_cols_dev(int engine, int D, const uint64_t* cw, size_t N, size_t M, 
     default: throw Error("leaf hash: extension degree must be 1, 2 or 3");
   }
   merkle_from(engine, nodes, N, N / 4, st, root_out);
+}
+
+// SHA3-256 / Keccak-256 trees with their top on the host.  The narrow levels of a Keccak
+// tree are one lone-wave Keccak-f each (≈ 15 us per level on the GPU, the permutation
+// being ~4600 dependent-issue instructions), against ≈ 0.2-0.7 us for a host
+// permutation: the launches stop at a level of at most KECCAK_HOST_TOP nodes, which comes
+// to the host through the mailbox (the wait the root needed anyway); the host hashes the
+// levels above it and copies them back (they are contiguous at the end of the array).
+static constexpr size_t KECCAK_HOST_TOP = 64;
+
+// One node: the sponge of the 64-byte message l || r (rate 136 bytes: one permutation,
+// the domain byte `pad` right after the message and 0x80 in the last byte of the rate).
+static void keccak_pair_host(uint8_t pad, const uint8_t* l, const uint8_t* r, uint8_t* out) {
+  uint64_t A[25] = {};
+  memcpy(A, l, 32);
+  memcpy(A + 4, r, 32);
+  A[8] ^= pad;
+  A[16] ^= 0x80ull << 56;
+  keccak_f1600(A);
+  memcpy(out, A, 32);
+}
+
+bool host_top_ok(int engine) { return engine == ENG_SHA3 || engine == ENG_KECCAK; }
+
+void commit_cols_host_top(int engine, int D, const uint64_t* cw, size_t N, size_t M, uint8_t* nodes, hipStream_t st,
+                          uint8_t root[32]) {
+  WHIR_REQUIRE(host_top_ok(engine), "commit: the host tree top is for the Keccak engines");
+  WHIR_REQUIRE(engine_supports_size(engine, 8 * M * D), "leaf hash: engine does not support the row size");
+  WHIR_REQUIRE(N > 0, "commit: empty codeword");
+  hash_rows_cols_dev(engine, D, cw, N, M, nodes, st);
+  const size_t W = (size_t)1 << merkle_layers(N);
+  if (W > N) fill_zero(nodes + 32 * N, 32 * (W - N), st);
+  const size_t w = merkle_from(engine, nodes, W, W, st, nullptr, {}, 1, KECCAK_HOST_TOP);
+  uint8_t* mb = (uint8_t*)result_mailbox(st);
+  WHIR_REQUIRE(32 * (2 * w - 1) <= 8 * RESULT_MAILBOX_WORDS, "commit: tree top too wide for the mailbox");
+  mailbox_arm(st);
+  WHIR_HIP(hipMemcpyAsync(mb, nodes + 32 * (2 * W - 2 * w), 32 * w, hipMemcpyDefault, st));
+  mailbox_wait(st);
+  const uint8_t pad = engine == ENG_SHA3 ? 0x06 : 0x01;
+  const uint8_t* cur = mb;
+  uint8_t* out = mb + 32 * w;
+  for (size_t width = w; width > 1; width >>= 1) {
+    for (size_t i = 0; i < width / 2; i++) keccak_pair_host(pad, cur + 64 * i, cur + 64 * i + 32, out + 32 * i);
+    cur = out;
+    out += 32 * (width / 2);
+  }
+  memcpy(root, cur, 32);
+  // (ordered before any later launch that writes results to the mailbox)
+  if (w > 1) WHIR_HIP(hipMemcpyAsync(nodes + 32 * (2 * W - w), mb + 32 * w, 32 * (w - 1), hipMemcpyDefault, st));
 }
 
 void commit_cols_many_dev(int engine, int D, const uint64_t* const* cw_tab, uint8_t* const* nodes_tab, size_t count,

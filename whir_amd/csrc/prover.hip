@@ -264,6 +264,8 @@ void irs_commit(Transcript& t, const IrsCfg& c, int engine, const uint64_t* cons
   const bool const_rows = L == 1 && (N & (N - 1)) == 0;  // every column is a constant: all rows are equal
   if (const_rows && const_rows_on_host(engine, 8 * M * Ds, N)) {
     commit_const_rows_host(Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, root);
+  } else if (!const_rows && host_top_ok(engine)) {
+    commit_cols_host_top(engine, Ds, w.cw.as<uint64_t>(), N, M, w.nodes.as<uint8_t>(), s, root);
   } else {
     // The root lands in the stream's pinned mailbox from the last tree launch.
     mailbox_arm(s);
