@@ -130,12 +130,16 @@ def test_merkle_commit_matches_oracle(engine, n):
     assert W.merkle_commit(engine, leaves) == O.merkle_commit(engine, leaves)
 
 
-@pytest.mark.parametrize("engine,n", [(W.HASH_BLAKE3, 1 << 19), (W.HASH_BLAKE3, (1 << 20) + 3),
+@pytest.mark.parametrize("engine,n", [(W.HASH_BLAKE3, 1 << 19), (W.HASH_BLAKE3, 1 << 20),
+                                      (W.HASH_BLAKE3, (1 << 20) + 3), (W.HASH_BLAKE3, (1 << 21) + 9),
                                       (W.HASH_SHA3, (1 << 22) + 5)])
 def test_merkle_commit_wide_levels_match_oracle(engine, n):
     """Trees with levels of 2^19 nodes and more: three-level launches of eight nodes
     per lane (2^20 -> 2^17, and for the 2^23-wide padded tree 2^23 -> 2^20 -> 2^17),
-    then the block-wide subtree kernel above them."""
+    then the block-wide subtree launches above them (for BLAKE3, one quad of lanes per
+    node once a launch has at most 2^16 lanes: 2^22 -> 2^19 -> 2^16 and then quads for
+    the 2^21 + 9 tree; the 2^20 + 3 tree's launch from 2^21 -> 2^18 still takes one
+    lane per node)."""
     rng = np.random.default_rng(n)
     leaves = rng.integers(0, 256, size=32 * n, dtype=np.uint8).tobytes()
     assert W.merkle_commit(engine, leaves) == O.merkle_commit(engine, leaves)

@@ -368,6 +368,166 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_sub(uint8_t* __restri
   }
 }
 
+// ------------------------------------------------------------------ BLAKE3, quad-split
+// Below 2^16 nodes a Merkle level holds less than one wave per SIMD, so each level
+// costs the latency of one wave running a whole compression.  Here the four lanes of a
+// quad share one compression: lane q holds column q of the state (v[q], v[4+q],
+// v[8+q], v[12+q]) and runs that column's G; for the diagonal step rows 1-3 rotate by
+// 1, 2 and 3 lanes inside the quad (DPP quad permutes), lane q runs diagonal q's G, and
+// the rows rotate back.  Lane q then owns chaining words q and 4+q.  The message words
+// each lane needs in each round are fixed by the schedule, so every lane reads its own
+// 28 words from LDS at offsets computed once per kernel.
+struct B3QSched {
+  uint16_t p[28];  // per (round, word slot): the message word of lane q in bits 4q..4q+3
+};
+constexpr B3QSched b3q_sched() {
+  constexpr int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+  int s[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};  // round word j = m[s[j]]
+  B3QSched t{};
+  for (int r = 0; r < 7; r++) {
+    for (int q = 0; q < 4; q++) {
+      t.p[4 * r + 0] |= (uint16_t)(s[2 * q] << (4 * q));
+      t.p[4 * r + 1] |= (uint16_t)(s[2 * q + 1] << (4 * q));
+      t.p[4 * r + 2] |= (uint16_t)(s[8 + 2 * q] << (4 * q));
+      t.p[4 * r + 3] |= (uint16_t)(s[9 + 2 * q] << (4 * q));
+    }
+    int n[16] = {};
+    for (int i = 0; i < 16; i++) n[i] = s[perm[i]];
+    for (int i = 0; i < 16; i++) s[i] = n[i];
+  }
+  return t;
+}
+
+// Lane q of the quad receives v from lane ((CTRL >> 2q) & 3) of the same quad.
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+
+// BLAKE3 of the 64-byte message msg[0..15] (LDS, the same for the quad's four lanes),
+// flags CHUNK_START | CHUNK_END | ROOT.  All four lanes of the quad must be active.
+__device__ __forceinline__ void b3q_pair(const uint32_t* msg, const uint32_t (&mo)[28], uint32_t iva, uint32_t ivb,
+                                         uint32_t dinit, uint32_t& lo, uint32_t& hi) {
+  using hd::rotr32;
+  uint32_t x[28];
+#pragma unroll
+  for (int j = 0; j < 28; j++) x[j] = msg[mo[j]];
+  uint32_t a = iva, b = ivb, c = iva, d = dinit;
+#pragma unroll
+  for (int r = 0; r < 7; r++) {
+    B3G(a, b, c, d, x[4 * r], x[4 * r + 1]);
+    b = quad_perm<0x39>(b);  // lanes q+1, q+2, q+3
+    c = quad_perm<0x4E>(c);
+    d = quad_perm<0x93>(d);
+    B3G(a, b, c, d, x[4 * r + 2], x[4 * r + 3]);
+    b = quad_perm<0x93>(b);  // back: lanes q-1, q-2, q-3
+    c = quad_perm<0x4E>(c);
+    d = quad_perm<0x39>(d);
+  }
+  lo = a ^ c;
+  hi = b ^ d;
+}
+
+// k_merkle_sub for BLAKE3 with a quad per node: quad i takes the place of lane i (the
+// same phases, groups and node indices), four times the threads.
+constexpr int B3Q_THREADS = 4 * MERKLE_THREADS;
+
+__global__ __launch_bounds__(B3Q_THREADS) void k_merkle_sub_b3q(uint8_t* __restrict__ nodes, size_t W, size_t w,
+                                                                 int L, int G1, uint32_t* root_out, BatchTab tab,
+                                                                 MailboxSig sig) {
+  __shared__ uint32_t buf[2][MERKLE_THREADS][8];
+  __shared__ uint32_t grp[MERKLE_THREADS][8][8];
+  size_t blk = blockIdx.x;
+  if (tab.nodes) {
+    nodes = tab.nodes[blk / tab.per];
+    if (root_out) root_out = (uint32_t*)(tab.roots + 32 * (blk / tab.per));
+    blk %= tab.per;
+  }
+  uint32_t* nodes32 = (uint32_t*)nodes;
+  const uint32_t q = threadIdx.x & 3, qi = threadIdx.x >> 2;
+  constexpr B3QSched S = b3q_sched();
+  uint32_t mo[28];
+#pragma unroll
+  for (int j = 0; j < 28; j++) mo[j] = (S.p[j] >> (4 * q)) & 15u;
+  const uint32_t iva = hd::B3_IV[q], ivb = hd::B3_IV[4 + q];
+  const uint32_t dinit = q == 2 ? 64u : (q == 3 ? (uint32_t)(hd::B3_CHUNK_START | hd::B3_CHUNK_END | hd::B3_ROOT) : 0u);
+  size_t width = w;
+  size_t first = blk << L;
+  size_t cnt = (size_t)1 << L;
+  int left = L, cur = 0;
+  bool from_global = true;
+  while (left > 0) {
+    const int G = from_global ? (left < G1 ? left : G1) : 1;
+    const size_t quads = cnt >> G;
+    if (qi < quads) {
+      const size_t f = first + ((size_t)qi << G);
+      uint32_t lo = 0, hi = 0;
+      if (from_global) {
+        for (int k = (int)q; k < (1 << G); k += 4) {
+          const uint4* src = (const uint4*)(nodes + 32 * (2 * W - 2 * width + f + k));
+          uint4* dst = (uint4*)&grp[qi][k][0];
+          dst[0] = src[0];
+          dst[1] = src[1];
+        }
+        __builtin_amdgcn_wave_barrier();
+        size_t wd = width, fl = f;
+        for (int lv = 0; lv < G; lv++) {
+          wd >>= 1;
+          fl >>= 1;
+          const size_t base = 2 * W - 2 * wd + fl;
+          for (int k = 0; k < (1 << (G - 1 - lv)); k++) {
+            b3q_pair(&grp[qi][2 * k][0], mo, iva, ivb, dinit, lo, hi);
+            __builtin_amdgcn_wave_barrier();
+            grp[qi][k][q] = lo;
+            grp[qi][k][4 + q] = hi;
+            nodes32[8 * (base + k) + q] = lo;
+            nodes32[8 * (base + k) + 4 + q] = hi;
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+      } else {
+        b3q_pair(&buf[cur][2 * qi][0], mo, iva, ivb, dinit, lo, hi);
+        const size_t base = 2 * W - 2 * (width >> 1) + (f >> 1);
+        nodes32[8 * base + q] = lo;
+        nodes32[8 * base + 4 + q] = hi;
+      }
+      buf[cur ^ 1][qi][q] = lo;
+      buf[cur ^ 1][qi][4 + q] = hi;
+    }
+    __syncthreads();
+    cur ^= 1;
+    from_global = false;
+    width >>= G;
+    first >>= G;
+    cnt = quads;
+    left -= G;
+  }
+  if (root_out && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) root_out[i] = buf[cur][0][i];
+    mailbox_raise(sig);
+  }
+}
+
+// Quad-split BLAKE3 levels on (WHIR_MERKLE_QUAD=0 keeps one lane per node, for comparison).
+static bool b3q_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("WHIR_MERKLE_QUAD");
+    return !e || atoi(e) != 0;
+  }();
+  return on;
+}
+// A launch takes the quad kernel while its first phase has at most this many lanes
+// (one wave per SIMD): there each level costs one compression's latency, which the
+// quads cut; on more lanes the launch is bound by instruction issue, and the quads'
+// extra instructions (DPP moves, per-lane message reads, four lanes of G) cost more
+// than they save.  Measured on MI355X per launch, pairs per lane (one lane / quad):
+// one block 16.0 / 10.1 us, 2^16 nodes (128 blocks) 19.2 / 12.9, 2^17 nodes 19.4 /
+// 13.6, 2^18 nodes (2^17 lanes) 25.1 / 26.0; eight nodes per lane from 2^19 nodes
+// (which k_merkle_wide takes first anyway) 38.4 us as quads against 32.2 us for
+// k_merkle_wide and the quad launch above it.
+static constexpr size_t B3Q_MAX_LANES = (size_t)1 << 16;
+
 // Three Merkle levels above a level of width w (w % 2^11 == 0): lane g reduces nodes
 // 8g..8g+7 to one in registers and stores the seven nodes it makes.  No lane idles
 // and nothing waits at a barrier, which matters on levels wide enough to keep every
@@ -429,17 +589,23 @@ static size_t merkle_from(int engine, uint8_t* nodes, size_t W, size_t w, hipStr
     int L = 0;
     while (L < maxL && ((size_t)2 << L) <= w && (w >> (L + 1)) >= stop) L++;
     const size_t blocks = w >> L;
-    const unsigned threads =
-        (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, ((size_t)1 << L) >> std::min(L, G1)));
-    ProfScope prof("k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)) * count);
+    const size_t lanes = ((size_t)1 << L) >> std::min(L, G1);  // nodes made by the first phase
+    const bool quad = engine == ENG_BLAKE3 && b3q_enabled() && blocks * count * lanes <= B3Q_MAX_LANES;
+    const unsigned threads = quad ? (unsigned)std::max<size_t>(64, std::min<size_t>(B3Q_THREADS, 4 * lanes))
+                                  : (unsigned)std::max<size_t>(64, std::min<size_t>(MERKLE_THREADS, lanes));
+    ProfScope prof(quad ? "k_merkle_sub_b3q" : "k_merkle_sub", st, 32.0 * (double)(2 * w - (w >> L)) * count);
     tab.per = (uint32_t)blocks;
     uint32_t* ro = blocks == 1 ? (tab.nodes ? (uint32_t*)tab.roots : (uint32_t*)root_out) : nullptr;
     // The launch that makes a single tree's root is the last of the commit.
     const MailboxSig sig = blocks == 1 && !tab.nodes && ro ? mailbox_take(ro) : MailboxSig{};
-    engine_switch(engine, [&](auto E) {
-      k_merkle_sub<decltype(E)::value><<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro, tab,
-                                                                                       sig);
-    });
+    if (quad) {
+      k_merkle_sub_b3q<<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro, tab, sig);
+    } else {
+      engine_switch(engine, [&](auto E) {
+        k_merkle_sub<decltype(E)::value><<<(unsigned)(blocks * count), threads, 0, st>>>(nodes, W, w, L, G1, ro,
+                                                                                         tab, sig);
+      });
+    }
     WHIR_HIP(hipGetLastError());
     w >>= L;
   }
